@@ -1,0 +1,379 @@
+// Bin-density scatter, density finaliser and field gather + combine.
+//
+// Compiled with -ffp-contract=off: the footprint and the per-(object, bin)
+// contribution reproduce the reference's float/double arithmetic exactly, so
+// the int64 fixed-point maps are bit-identical to the CPU oracle's.
+#include "ep_internal.h"
+#include "reduce.h"
+
+namespace ep {
+
+struct Foot {
+  float l, b, r, t, sx, sy;
+  int i0, i1, j0, j1;
+};
+
+// float_less (global.h:350-353): float difference compared in double.
+__device__ __forceinline__ bool f_less(float a, float b) { return (double)(a - b) < -kEps; }
+
+// Inflated rectangle, smoothing scales and clamped bin range of one object
+// (eplace.cpp:1042-1104 for the scatter, 601-659 for the gather).
+__device__ __forceinline__ Foot footprint(float cx, float cy, float w, float h, const Grid &g) {
+  Foot f;
+  f.sx = 1.f;
+  f.sy = 1.f;
+  f.l = cx - 0.5f * w;  // Module::setCenter_2D (objects.cpp:530-539)
+  f.b = cy - 0.5f * h;
+  f.r = f.l + w;        // Module::getUR_2D (objects.cpp:497-509)
+  f.t = f.b + h;
+  if (f_less(w, g.bsx)) {
+    f.sx = w / g.bsx;
+    f.l = (float)((double)cx - 0.5 * (double)g.bsx);
+    f.r = (float)((double)cx + 0.5 * (double)g.bsx);
+  }
+  if (f_less(h, g.bsy)) {
+    f.sy = h / g.bsy;
+    f.b = (float)((double)cy - 0.5 * (double)g.bsy);
+    f.t = (float)((double)cy + 0.5 * (double)g.bsy);
+  }
+  f.i0 = (int)((f.l - g.core_llx) / g.bsx);
+  f.i1 = (int)((f.r - g.core_llx) / g.bsx);
+  f.j0 = (int)((f.b - g.core_lly) / g.bsy);
+  f.j1 = (int)((f.t - g.core_lly) / g.bsy);
+  // hi clamp as the reference; lo clamp only guards non-finite inputs
+  f.i0 = max(f.i0, 0);
+  f.j0 = max(f.j0, 0);
+  f.i1 = min(f.i1, g.nx - 1);
+  f.j1 = min(f.j1, g.ny - 1);
+  return f;
+}
+
+// getOverlapArea (global.h:477-518).
+__device__ __forceinline__ double overlap_len(double x1, double x2, double x3, double x4) {
+  double s = fmax(x1, x3), e = fmin(x2, x4);
+  return s >= e ? 0.0 : s - e;
+}
+__device__ __forceinline__ double bin_overlap(const Grid &g, int i, int j, const Foot &f) {
+  double hl = overlap_len(g.bllx[i], g.burx[i], f.l, f.r);
+  if (hl == 0) return 0;
+  double vl = overlap_len(g.blly[j], g.bury[j], f.b, f.t);
+  if (vl == 0) return 0;
+  return hl * vl;
+}
+
+__global__ void zero_maps_kernel(int64_t n, unsigned long long *a, unsigned long long *b) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    a[i] = 0;
+    b[i] = 0;
+  }
+}
+
+// Addend of eplace.cpp:1116/1123/1127 for one bin, as int64 units of 2^-shift.
+__device__ __forceinline__ unsigned long long fixed_contribution(float s, double ov, double scale) {
+  float c = s * (float)ov;
+  return (unsigned long long)__double2ll_rn((double)c * scale);
+}
+
+__global__ void __launch_bounds__(kBlock) scatter_kernel(int O, const float *__restrict__ x,
+                                                          const float *__restrict__ y,
+                                                          const float *__restrict__ w,
+                                                          const float *__restrict__ h,
+                                                          const uint8_t *__restrict__ kind,
+                                                          const uint8_t *__restrict__ is_big, Grid g,
+                                                          double scale, unsigned long long *node_q,
+                                                          unsigned long long *filler_q) {
+  int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= O || is_big[o]) return;
+  uint8_t k = kind[o];
+  Foot f = footprint(x[o], y[o], w[o], h[o], g);
+  float s = f.sx * f.sy;
+  if (k == KIND_MACRO) s = s * g.td;
+  unsigned long long *dst = k == KIND_FILLER ? filler_q : node_q;
+  for (int i = f.i0; i <= f.i1; ++i)
+    for (int j = f.j0; j <= f.j1; ++j) {
+      double ov = bin_overlap(g, i, j, f);
+      if (ov == 0) continue;
+      atomicAdd(dst + (size_t)i * g.ny + j, fixed_contribution(s, ov, scale));
+    }
+}
+
+// Objects covering many bins (macros): one block each, threads over bins.
+__global__ void __launch_bounds__(kBlock) scatter_big_kernel(const int32_t *__restrict__ big,
+                                                              const float *__restrict__ x,
+                                                              const float *__restrict__ y,
+                                                              const float *__restrict__ w,
+                                                              const float *__restrict__ h,
+                                                              const uint8_t *__restrict__ kind, Grid g,
+                                                              double scale, unsigned long long *node_q,
+                                                              unsigned long long *filler_q) {
+  int o = big[blockIdx.x];
+  uint8_t k = kind[o];
+  Foot f = footprint(x[o], y[o], w[o], h[o], g);
+  float s = f.sx * f.sy;
+  if (k == KIND_MACRO) s = s * g.td;
+  unsigned long long *dst = k == KIND_FILLER ? filler_q : node_q;
+  int ni = f.i1 - f.i0 + 1, nj = f.j1 - f.j0 + 1;
+  if (ni <= 0 || nj <= 0) return;
+  int64_t total = (int64_t)ni * nj;
+  for (int64_t t = threadIdx.x; t < total; t += blockDim.x) {
+    int i = f.i0 + (int)(t / nj), j = f.j0 + (int)(t % nj);
+    double ov = bin_overlap(g, i, j, f);
+    if (ov == 0) continue;
+    atomicAdd(dst + (size_t)i * g.ny + j, fixed_contribution(s, ov, scale));
+  }
+}
+
+// Fixed-point maps -> float maps -> rho (eplace.cpp:571-580) and the overflow
+// sum (eplace.cpp:438-453); the last block finishes tau and the gamma
+// schedule (eplace.cpp:475-500).
+__global__ void __launch_bounds__(kBlock) finalize_density_kernel(
+    int nb, const unsigned long long *__restrict__ node_q, const unsigned long long *__restrict__ filler_q,
+    const float *__restrict__ term, const float *__restrict__ base, Grid g, int ldp, double inv_scale,
+    double node_area_scaled, float *__restrict__ rho, double *partials, unsigned int *ticket,
+    DevScalars *sc) {
+  if (sc->done) return;
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  double ovf = 0, rsum = 0;
+  if (k < nb) {
+    float node = (float)((double)(long long)node_q[k] * inv_scale);
+    float fill = (float)((double)(long long)filler_q[k] * inv_scale);
+    float t = term[k], b = base[k];
+    float e = node + b + fill + t;
+    int i = k / g.ny, j = k - i * g.ny;
+    float r = e * g.inv_bin_area;
+    rho[(size_t)i * ldp + j] = r;
+    rsum = r;
+    float d = (node + t + b) * g.inv_bin_area - g.td;
+    ovf = (double)(fmaxf(0.f, d) * g.bin_area);
+  }
+  double s = block_sum(ovf);
+  double rs = block_sum(rsum);
+  if (threadIdx.x == 0) {
+    partials[kPartialTau + blockIdx.x] = s;
+    partials[kPartialRho + blockIdx.x] = rs;
+  }
+  if (last_block_done(ticket)) {
+    double tot = block_sum_partials(partials + kPartialTau, gridDim.x);
+    double rtot = block_sum_partials(partials + kPartialRho, gridDim.x);
+    if (threadIdx.x == 0) {
+      double tau = tot / node_area_scaled;
+      sc->overflow = tau;
+      sc->rho_mean = rtot / (double)nb;
+      // 1/gamma in the reference's float arithmetic
+      float cx = 0.125f / g.bsx, cy = 0.125f / g.bsy;
+      float tf = (float)tau;
+      if (tf > 1.0) {
+        cx = (float)(cx * 0.1);
+        cy = (float)(cy * 0.1);
+      } else if (tf < 0.1) {
+        cx = (float)(cx * 10.0);
+        cy = (float)(cy * 10.0);
+      } else {
+        float temp = (float)(1.0 / pow(10.0, ((double)tf - 0.1) * 20 / 9.0 - 1.0));
+        cx *= temp;
+        cy *= temp;
+      }
+      sc->inv_gamma_x = cx;
+      sc->inv_gamma_y = cy;
+      *ticket = 0;
+    }
+  }
+}
+
+// Density gradient of one object: sum over its bins of sx*sy*overlap*E
+// (eplace.cpp:661-671; the product is rounded to float as there).
+__device__ __forceinline__ float2 gather_one(const Foot &f, const Grid &g, int ldp, const float *ex,
+                                             const float *ey, int i, int j) {
+  double ov = bin_overlap(g, i, j, f);
+  float a = (float)((double)(f.sx * f.sy) * ov);
+  size_t k = (size_t)i * ldp + j;
+  return make_float2(a * ex[k], a * ey[k]);
+}
+
+__global__ void __launch_bounds__(kBlock) gather_big_kernel(const int32_t *__restrict__ big,
+                                                             const float *__restrict__ x,
+                                                             const float *__restrict__ y,
+                                                             const float *__restrict__ w,
+                                                             const float *__restrict__ h, Grid g, int ldp,
+                                                             const float *__restrict__ ex,
+                                                             const float *__restrict__ ey, float2 *gd,
+                                                             const DevScalars *sc) {
+  if (sc->done) return;
+  int o = big[blockIdx.x];
+  Foot f = footprint(x[o], y[o], w[o], h[o], g);
+  int ni = f.i1 - f.i0 + 1, nj = f.j1 - f.j0 + 1;
+  double gx = 0, gy = 0;
+  if (ni > 0 && nj > 0) {
+    int64_t total = (int64_t)ni * nj;
+    for (int64_t t = threadIdx.x; t < total; t += blockDim.x) {
+      float2 v = gather_one(f, g, ldp, ex, ey, f.i0 + (int)(t / nj), f.j0 + (int)(t % nj));
+      gx += v.x;
+      gy += v.y;
+    }
+  }
+  gx = block_sum(gx);
+  gy = block_sum(gy);
+  if (threadIdx.x == 0) gd[o] = make_float2((float)gx, (float)gy);
+}
+
+// Gather (small objects), node wirelength-gradient reduction over the
+// node's pins in modulePins order (eplace.cpp:531-549), preconditioned
+// combine (eplace.cpp:694-737) and, optionally, the Lipschitz partial sums
+// |g - g_prev|^2 and |y - y_prev|^2 over the stage's objects
+// (optutils.hpp:19-35).
+__global__ void __launch_bounds__(kBlock) gather_combine_kernel(
+    int O, int N, const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ w,
+    const float *__restrict__ h, const uint8_t *__restrict__ kind, const uint8_t *__restrict__ is_big,
+    const int32_t *__restrict__ npins, const float *__restrict__ area, Grid g, int ldp,
+    const float *__restrict__ ex, const float *__restrict__ ey, const int32_t *__restrict__ node_pin_start,
+    const int32_t *__restrict__ node_pins, const float2 *__restrict__ pin_grad, float2 *gd, float2 *gwl,
+    float2 *gtot, int recompute, int lipschitz, const uint8_t *__restrict__ in_stage,
+    const float2 *__restrict__ ref_now, const float2 *__restrict__ ref_prev,
+    const float2 *__restrict__ grad_prev, double *partials, const DevScalars *sc) {
+  if (sc->done) return;
+  int o = blockIdx.x * blockDim.x + threadIdx.x;
+  double ln = 0, ld = 0;
+  if (o < O) {
+    uint8_t k = kind[o];
+    float2 d;
+    if (recompute && !is_big[o]) {
+      Foot f = footprint(x[o], y[o], w[o], h[o], g);
+      float gx = 0, gy = 0;
+      for (int i = f.i0; i <= f.i1; ++i)
+        for (int j = f.j0; j <= f.j1; ++j) {
+          float2 v = gather_one(f, g, ldp, ex, ey, i, j);
+          gx += v.x;
+          gy += v.y;
+        }
+      d = make_float2(gx, gy);
+      gd[o] = d;
+    } else {
+      d = gd[o];
+    }
+    float2 wl = make_float2(0.f, 0.f);
+    if (o < N) {
+      if (recompute) {
+        int s = node_pin_start[o], e = node_pin_start[o + 1];
+        for (int q = s; q < e; ++q) {
+          float2 pg = pin_grad[node_pins[q]];
+          wl.x += pg.x;
+          wl.y += pg.y;
+        }
+        gwl[o] = wl;
+      } else {
+        wl = gwl[o];
+      }
+    }
+    float lam = (float)sc->lambda;
+    float pre = 1.f / fmaxf(1.0f, (float)npins[o] + lam * area[o]);
+    float2 t;
+    if (k == KIND_FILLER) {
+      t.x = pre * lam * d.x;
+      t.y = pre * lam * d.y;
+    } else {
+      t.x = pre * (lam * d.x - wl.x);
+      t.y = pre * (lam * d.y - wl.y);
+    }
+    gtot[o] = t;
+    if (lipschitz && in_stage[o]) {
+      float2 gp = grad_prev[o];
+      float2 rn = ref_now ? ref_now[o] : make_float2(x[o], y[o]);
+      float2 rp = ref_prev[o];
+      double dgx = (double)t.x - gp.x, dgy = (double)t.y - gp.y;
+      double dxx = (double)rn.x - rp.x, dxy = (double)rn.y - rp.y;
+      ln = dgx * dgx + dgy * dgy;
+      ld = dxx * dxx + dxy * dxy;
+    }
+  }
+  if (lipschitz) {
+    ln = block_sum(ln);
+    ld = block_sum(ld);
+    if (threadIdx.x == 0) {
+      partials[kPartialLip + 2 * blockIdx.x] = ln;
+      partials[kPartialLip + 2 * blockIdx.x + 1] = ld;
+    }
+  }
+}
+
+// lambda0 = sum |gwl| / sum |gd| (eplace.cpp:812-835), partial sums.
+__global__ void __launch_bounds__(kBlock) lambda_init_kernel(int O, int N, const float2 *__restrict__ gd,
+                                                              const float2 *__restrict__ gwl,
+                                                              double *partials, unsigned int *ticket,
+                                                              DevScalars *sc) {
+  int o = blockIdx.x * blockDim.x + threadIdx.x;
+  double a = 0, b = 0;
+  if (o < O) {
+    float2 d = gd[o];
+    b = fabs((double)d.x) + fabs((double)d.y);
+    if (o < N) {
+      float2 wv = gwl[o];
+      a = fabs((double)wv.x) + fabs((double)wv.y);
+    }
+  }
+  a = block_sum(a);
+  b = block_sum(b);
+  if (threadIdx.x == 0) {
+    partials[kPartialLambda + 2 * blockIdx.x] = a;
+    partials[kPartialLambda + 2 * blockIdx.x + 1] = b;
+  }
+  if (last_block_done(ticket)) {
+    double na = block_sum_partials_strided(partials + kPartialLambda, gridDim.x, 2);
+    double nb = block_sum_partials_strided(partials + kPartialLambda + 1, gridDim.x, 2);
+    if (threadIdx.x == 0) {
+      sc->lambda = na / nb;
+      sc->last_hpwl = sc->hpwl;
+      *ticket = 0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+static int blocks_for(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
+
+void launch_zero_maps(Ctx &c) {
+  int64_t nb = (int64_t)c.nx * c.ny;
+  hipLaunchKernelGGL(zero_maps_kernel, dim3(blocks_for(nb)), dim3(kBlock), 0, c.stream, nb, c.node_q,
+                     c.filler_q);
+}
+
+void launch_scatter(Ctx &c) {
+  Grid g = c.grid();
+  double scale = ldexp(1.0, c.shift);
+  hipLaunchKernelGGL(scatter_kernel, dim3(blocks_for(c.O)), dim3(kBlock), 0, c.stream, c.O, c.x, c.y, c.w, c.h,
+                     c.kind, c.is_big, g, scale, c.node_q, c.filler_q);
+  if (c.num_big)
+    hipLaunchKernelGGL(scatter_big_kernel, dim3(c.num_big), dim3(kBlock), 0, c.stream, c.big, c.x, c.y, c.w,
+                       c.h, c.kind, g, scale, c.node_q, c.filler_q);
+}
+
+int poisson_ld(const Ctx &c);
+
+void launch_finalize_density(Ctx &c) {
+  Grid g = c.grid();
+  int nb = c.nx * c.ny;
+  double nas = (double)(c.std_area + c.macro_area * c.td);
+  hipLaunchKernelGGL(finalize_density_kernel, dim3(blocks_for(nb)), dim3(kBlock), 0, c.stream, nb, c.node_q,
+                     c.filler_q, c.term_map, c.base_map, g, poisson_ld(c), ldexp(1.0, -c.shift), nas, c.rho,
+                     c.partials, c.tickets + 0, c.sc);
+}
+
+void launch_gather_combine(Ctx &c, bool recompute, bool lipschitz, const float2 *ref_now,
+                           const float2 *ref_prev, const float2 *grad_prev) {
+  Grid g = c.grid();
+  int ldp = poisson_ld(c);
+  if (recompute && c.num_big)
+    hipLaunchKernelGGL(gather_big_kernel, dim3(c.num_big), dim3(kBlock), 0, c.stream, c.big, c.x, c.y, c.w, c.h,
+                       g, ldp, c.ex, c.ey, c.gd, c.sc);
+  hipLaunchKernelGGL(gather_combine_kernel, dim3(blocks_for(c.O)), dim3(kBlock), 0, c.stream, c.O, c.N, c.x,
+                     c.y, c.w, c.h, c.kind, c.is_big, c.npins, c.area, g, ldp, c.ex, c.ey, c.node_pin_start,
+                     c.node_pins, c.pin_grad, c.gd, c.gwl, c.gtot, recompute ? 1 : 0, lipschitz ? 1 : 0,
+                     c.in_stage, ref_now, ref_prev, grad_prev, c.partials, c.sc);
+}
+
+void launch_lambda_init(Ctx &c) {
+  hipLaunchKernelGGL(lambda_init_kernel, dim3(blocks_for(c.O)), dim3(kBlock), 0, c.stream, c.O, c.N, c.gd,
+                     c.gwl, c.partials, c.tickets + 1, c.sc);
+}
+
+}  // namespace ep

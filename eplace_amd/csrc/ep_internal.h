@@ -1,0 +1,182 @@
+// Internal declarations shared by the engine's HIP sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/eplace_amd.h"
+
+namespace ep {
+
+enum ObjKind : uint8_t { KIND_CELL = 0, KIND_MACRO = 1, KIND_FILLER = 2 };
+
+constexpr double kEps = 1.0e-15;  // global.h:39
+constexpr int kBlock = 256;
+constexpr int kSmallNetMaxDegree = 32;  // nets above this go to the wave-per-net kernel
+constexpr int kBigObjectBins = 16;      // objects that may cover more bins get a block each
+
+// Bin grid, passed by value to the density kernels.  Edges are the
+// reference's float bin coordinates (eplace.cpp:292-314).
+struct Grid {
+  int nx, ny;
+  float bsx, bsy, core_llx, core_lly;
+  float bin_area, inv_bin_area, td;
+  const float *bllx, *burx, *blly, *bury;
+};
+
+// Device-resident scalars of the evaluator and optimiser.
+struct DevScalars {
+  double overflow;     // tau of the last scatter
+  double inv_gamma_x, inv_gamma_y;
+  double lambda;
+  double hpwl;         // HPWL at the positions of the last wirelength pass
+  double last_hpwl;
+  double wl_value;
+  double step, a_k, a_next, coef, new_step;
+  double lip_num, lip_den;
+  double init_lambda_num, init_lambda_den;
+  double rho_mean;     // mean charge density; removed before the transform (DC is discarded)
+  int iter, done, accepted, stage;
+  int evals, backtracks, max_iter, bad_numeric;
+  float target_overflow, pad0;
+};
+
+// Slots of the partial-sum buffer (doubles).
+constexpr int kPartialTau = 0;          // up to kMaxPartials
+constexpr int kMaxPartials = 32768;
+constexpr int kPartialWL = kMaxPartials;        // hpwl, wl pairs
+constexpr int kPartialLip = 3 * kMaxPartials;   // num, den pairs
+constexpr int kPartialLambda = 5 * kMaxPartials;  // |gwl|, |gd| pairs
+constexpr int kPartialRho = 7 * kMaxPartials;      // sum of rho per block
+constexpr int kPartialTotal = 8 * kMaxPartials;
+
+struct Poisson;  // poisson.hip
+
+struct Ctx {
+  std::string err;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[6] = {};
+  double last_timings[5] = {};
+
+  // sizes
+  int N = 0, F = 0, O = 0, NT = 0, E = 0;
+  int64_t P = 0;
+  int nx = 0, ny = 0;
+  int shift = 0;
+  int stage = EP_STAGE_MGP;
+  bool eval_valid = false;  // gd/gwl/tau describe the current positions
+  int wl_model = 0, exp_mode = 0, poisson_mode = 0;
+  float td = 1.f;
+
+  // host copies needed for stage handling and filler re-randomisation
+  std::vector<float> h_w, h_h;
+  std::vector<uint8_t> h_kind;
+  std::vector<int32_t> h_stage_idx;  // objects of the current stage, in stage order
+  float core_llx = 0, core_lly = 0, core_urx = 0, core_ury = 0, total_row_area = 0;
+  float bsx = 0, bsy = 0;
+  float std_area = 0, macro_area = 0, filler_w = 0, filler_h = 0;
+  struct Rand *rng = nullptr;
+  double lambda_host_init = 0;
+
+  // device: objects [O]
+  float *x = nullptr, *y = nullptr, *w = nullptr, *h = nullptr, *area = nullptr;
+  uint8_t *kind = nullptr;
+  uint8_t *in_stage = nullptr;
+  int32_t *npins = nullptr;
+  int32_t *big = nullptr;   // objects handled by the block-per-object kernels
+  int num_big = 0;
+  uint8_t *is_big = nullptr;
+  // terminals
+  float *tx = nullptr, *ty = nullptr;
+  // nets / pins
+  int32_t *net_start = nullptr;  // [E+1]
+  int32_t *pin_obj = nullptr;
+  float *pin_dx = nullptr, *pin_dy = nullptr;
+  int32_t *small_nets = nullptr, *large_nets = nullptr;
+  int num_small = 0, num_large = 0;
+  int32_t *node_pin_start = nullptr;  // [N+1]
+  int32_t *node_pins = nullptr;
+  float2 *pin_grad = nullptr;
+  // bins
+  float *bllx = nullptr, *burx = nullptr, *blly = nullptr, *bury = nullptr;
+  unsigned long long *node_q = nullptr, *filler_q = nullptr;
+  float *term_map = nullptr, *base_map = nullptr;
+  float *rho = nullptr, *phi = nullptr, *ex = nullptr, *ey = nullptr;
+  // gradients [O]
+  float2 *gd = nullptr, *gwl = nullptr, *gtot = nullptr;
+  // optimiser buffers [O] (float2)
+  float2 *cur_main = nullptr, *cur_ref = nullptr, *last_ref = nullptr, *last_grad = nullptr;
+  float2 *new_main = nullptr, *new_ref = nullptr, *cur_grad = nullptr;
+  // scalars
+  DevScalars *sc = nullptr;
+  double *partials = nullptr;
+  unsigned int *tickets = nullptr;
+  Poisson *poisson = nullptr;
+
+  Grid grid() const {
+    Grid g;
+    g.nx = nx;
+    g.ny = ny;
+    g.bsx = bsx;
+    g.bsy = bsy;
+    g.core_llx = core_llx;
+    g.core_lly = core_lly;
+    g.bin_area = bsx * bsy;
+    g.inv_bin_area = (float)(1.0 / (double)(bsx * bsy));
+    g.td = td;
+    g.bllx = bllx;
+    g.burx = burx;
+    g.blly = blly;
+    g.bury = bury;
+    return g;
+  }
+};
+
+// ---- launchers (density.hip) ----
+void launch_zero_maps(Ctx &c);
+void launch_scatter(Ctx &c);
+void launch_finalize_density(Ctx &c);
+void launch_gather_combine(Ctx &c, bool recompute_gather, bool lipschitz, const float2 *ref_now,
+                           const float2 *ref_prev, const float2 *grad_prev);
+// ---- wirelength.hip ----
+void launch_wirelength(Ctx &c, bool gradient);
+void launch_reduce_wirelength(Ctx &c, int lambda_update);
+// ---- poisson.hip ----
+bool poisson_create(Ctx &c, std::string &err);
+void poisson_destroy(Ctx &c);
+void launch_poisson(Ctx &c, bool with_phi);
+// ---- nesterov.hip ----
+void launch_positions_to_ref(Ctx &c, float2 *dst);
+void launch_step_scalar(Ctx &c, int bktrk_first);
+void launch_nesterov_move(Ctx &c, const float2 *ref, const float2 *grad, const float2 *main_in,
+                          float2 *main_out, float2 *ref_unclamped_out);
+void launch_accept(Ctx &c);
+void launch_lambda_init(Ctx &c);
+void launch_set_positions(Ctx &c, const float2 *xy_stage, const int32_t *stage_idx, int64_t n);
+void launch_copy_float2(Ctx &c, float2 *dst, const float2 *src, int64_t n);
+void launch_write_main_back(Ctx &c, const float2 *main_sol);
+void launch_done_check(Ctx &c, int next_iter);
+void launch_done_check_ex(Ctx &c, int next_iter, double *log, int log_cap, int set_iter);
+void launch_nesterov_move_ex(Ctx &c, const float2 *ref, const float2 *grad, const float2 *main_in,
+                             float2 *main_out, float2 *ref_unclamped_out, float2 *save_ref, float2 *save_grad);
+void launch_count_eval(Ctx &c);
+int poisson_ld(const Ctx &c);
+
+#define EP_HIP(call)                                                           \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess) {                                                    \
+      c.err = std::string(#call) + ": " + hipGetErrorString(e_);              \
+      return EP_ERR_HIP;                                                       \
+    }                                                                          \
+  } while (0)
+
+}  // namespace ep
+
+struct ep_ctx {
+  ep::Ctx c;
+};

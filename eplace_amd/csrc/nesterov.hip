@@ -1,0 +1,203 @@
+// Device-resident Nesterov optimiser pieces (Optimization/nesterov.hpp,
+// optutils.hpp) and the position setter (PlaceDB::setModuleCenter_2D,
+// placedb.cpp:253-288).  Compiled with -ffp-contract=off so the element
+// update rounds like the reference's VECTOR_3D arithmetic.
+#include "ep_internal.h"
+#include "reduce.h"
+
+namespace ep {
+
+struct Core {
+  float llx, lly, urx, ury;
+};
+
+// PlaceDB::setModuleCenter_2D clamp (placedb.cpp:253-288): compares in double,
+// +/- EPS, result stored as float.
+__device__ __forceinline__ float2 clamp_center(float x, float y, float w, float h, const Core &k) {
+  double hw = 0.5 * (double)w, hh = 0.5 * (double)h;
+  if ((double)x - hw < (double)k.llx) x = (float)((double)k.llx + hw + kEps);
+  if ((double)x + hw > (double)k.urx) x = (float)((double)k.urx - hw - kEps);
+  if ((double)y - hh < (double)k.lly) y = (float)((double)k.lly + hh + kEps);
+  if ((double)y + hh > (double)k.ury) y = (float)((double)k.ury - hh - kEps);
+  return make_float2(x, y);
+}
+
+__global__ void set_positions_kernel(int64_t n, const float2 *__restrict__ xy, const int32_t *__restrict__ idx,
+                                     float *x, float *y, const float *__restrict__ w,
+                                     const float *__restrict__ h, Core k) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int o = idx[t];
+  float2 c = clamp_center(xy[t].x, xy[t].y, w[o], h[o], k);
+  x[o] = c.x;
+  y[o] = c.y;
+}
+
+__global__ void positions_to_ref_kernel(int O, const float *__restrict__ x, const float *__restrict__ y,
+                                        float2 *dst) {
+  int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o < O) dst[o] = make_float2(x[o], y[o]);
+}
+
+__global__ void copy_float2_kernel(int64_t n, float2 *dst, const float2 *src) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+// Step length 1/L from the Lipschitz partial sums (optutils.hpp:19-35), the
+// momentum a_{k+1} = (1 + sqrt(4 a_k^2 + 1))/2 and (a_k - 1)/a_{k+1}
+// (nesterov.hpp:375-415).  Iteration 0 uses step 1.  A non-finite 1/L
+// (identical consecutive positions, 0/0) keeps the previous step and is
+// counted in bad_numeric.
+__global__ void __launch_bounds__(kBlock) step_scalar_kernel(int nblocks, double *partials, DevScalars *sc) {
+  if (sc->done) return;
+  double num = block_sum_partials_strided(partials + kPartialLip, nblocks, 2);
+  double den = block_sum_partials_strided(partials + kPartialLip + 1, nblocks, 2);
+  if (threadIdx.x == 0) {
+    double step = 1.0;
+    if (sc->iter > 0) {
+      double L = sqrt(num) / sqrt(den);
+      step = 1.0 / L;
+      if (!isfinite(step) || step <= 0.0) {
+        sc->bad_numeric += 1;
+        step = sc->step;
+      }
+    }
+    sc->lip_num = num;
+    sc->lip_den = den;
+    sc->step = step;
+    double a = sc->a_k;
+    double an = (1.0 + sqrt(4.0 * a * a + 1.0)) / 2.0;
+    sc->a_next = an;  // committed by done_check
+    sc->coef = (a - 1.0) / an;
+  }
+}
+
+// Backtracking acceptance (nesterov.hpp:337-364): accept when
+// 0.95 * step <= 1/L', L' from the trial evaluation; step := 1/L' either way.
+__global__ void __launch_bounds__(kBlock) accept_kernel(int nblocks, double *partials, DevScalars *sc) {
+  if (sc->done) return;
+  double num = block_sum_partials_strided(partials + kPartialLip, nblocks, 2);
+  double den = block_sum_partials_strided(partials + kPartialLip + 1, nblocks, 2);
+  if (threadIdx.x == 0) {
+    double ns = 1.0 / (sqrt(num) / sqrt(den));
+    if (!isfinite(ns) || ns <= 0.0) {
+      sc->bad_numeric += 1;
+      sc->accepted = 1;
+      return;
+    }
+    int ok = 0.95 * sc->step <= ns;
+    sc->accepted = ok;
+    if (!ok) sc->backtracks += 1;
+    sc->step = ns;
+  }
+}
+
+// x_{k+1} = y_k + step g_k;  y_{k+1} = x_{k+1} + coef (x_{k+1} - x_k);
+// positions := clamp(y_{k+1}) (nesterov.hpp:400-411 + setPosition).
+// ref == nullptr reads y_k from the positions; save_ref/save_grad receive
+// y_k and g_k for the next Lipschitz estimate.
+__global__ void __launch_bounds__(kBlock) nesterov_move_kernel(
+    int O, const uint8_t *__restrict__ in_stage, const float2 *ref, const float2 *__restrict__ grad,
+    const float2 *main_in, float2 *main_out, float2 *ref_unclamped_out, float2 *save_ref, float2 *save_grad,
+    float *x, float *y, const float *__restrict__ w, const float *__restrict__ h, Core k, const DevScalars *sc) {
+  if (sc->done) return;
+  int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= O || !in_stage[o]) return;
+  float step = (float)sc->step, coef = (float)sc->coef;
+  float2 r = ref ? ref[o] : make_float2(x[o], y[o]);
+  float2 g = grad[o];
+  float2 m = main_in[o];
+  if (save_ref) save_ref[o] = r;
+  if (save_grad) save_grad[o] = g;
+  float npx = r.x + g.x * step, npy = r.y + g.y * step;
+  float nrx = npx + (npx - m.x) * coef, nry = npy + (npy - m.y) * coef;
+  main_out[o] = make_float2(npx, npy);
+  if (ref_unclamped_out) ref_unclamped_out[o] = make_float2(nrx, nry);
+  float2 c = clamp_center(nrx, nry, w[o], h[o], k);
+  x[o] = c.x;
+  y[o] = c.y;
+}
+
+// End of an iteration: commit a_{k+1}, advance the counter, evaluate the
+// stop condition for the next iteration (nesterov.hpp:86-149) with the tau
+// of the last evaluation, append a log row.
+__global__ void done_check_kernel(DevScalars *sc, int next_iter, double *log, int log_cap, int set_iter) {
+  if (sc->done) return;
+  int k = next_iter - 1;
+  if (log && k >= 0 && k < log_cap) {
+    double *r = log + (size_t)k * 8;
+    r[0] = k;
+    r[1] = sc->overflow;
+    r[2] = sc->lambda;
+    r[3] = sc->hpwl;
+    r[4] = sc->step;
+    r[5] = sc->inv_gamma_x;
+    r[6] = sc->evals;
+    r[7] = sc->backtracks;
+  }
+  if (set_iter) {
+    sc->a_k = sc->a_next;
+    sc->iter = next_iter;
+  }
+  bool stop;
+  if (sc->stage == EP_STAGE_MGP) stop = ((float)sc->overflow < sc->target_overflow) || (next_iter > sc->max_iter);
+  else if (sc->stage == EP_STAGE_FILLERONLY) stop = next_iter >= 20;
+  else stop = ((float)sc->overflow < 0.07f) || (next_iter > sc->max_iter);
+  if (stop) sc->done = 1;
+}
+
+__global__ void count_eval_kernel(DevScalars *sc) {
+  if (!sc->done) sc->evals += 1;
+}
+
+static int blocks_for(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
+
+static Core core_of(const Ctx &c) { return Core{c.core_llx, c.core_lly, c.core_urx, c.core_ury}; }
+
+void launch_set_positions(Ctx &c, const float2 *xy, const int32_t *idx, int64_t n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(set_positions_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, c.stream, n, xy, idx, c.x, c.y,
+                     c.w, c.h, core_of(c));
+}
+
+void launch_positions_to_ref(Ctx &c, float2 *dst) {
+  hipLaunchKernelGGL(positions_to_ref_kernel, dim3(blocks_for(c.O)), dim3(kBlock), 0, c.stream, c.O, c.x, c.y,
+                     dst);
+}
+
+void launch_copy_float2(Ctx &c, float2 *dst, const float2 *src, int64_t n) {
+  hipLaunchKernelGGL(copy_float2_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, c.stream, n, dst, src);
+}
+
+void launch_step_scalar(Ctx &c, int) {
+  hipLaunchKernelGGL(step_scalar_kernel, dim3(1), dim3(kBlock), 0, c.stream, blocks_for(c.O), c.partials, c.sc);
+}
+
+void launch_accept(Ctx &c) {
+  hipLaunchKernelGGL(accept_kernel, dim3(1), dim3(kBlock), 0, c.stream, blocks_for(c.O), c.partials, c.sc);
+}
+
+void launch_nesterov_move_ex(Ctx &c, const float2 *ref, const float2 *grad, const float2 *main_in,
+                             float2 *main_out, float2 *ref_unclamped_out, float2 *save_ref, float2 *save_grad) {
+  hipLaunchKernelGGL(nesterov_move_kernel, dim3(blocks_for(c.O)), dim3(kBlock), 0, c.stream, c.O, c.in_stage,
+                     ref, grad, main_in, main_out, ref_unclamped_out, save_ref, save_grad, c.x, c.y, c.w, c.h,
+                     core_of(c), c.sc);
+}
+
+void launch_nesterov_move(Ctx &c, const float2 *ref, const float2 *grad, const float2 *main_in,
+                          float2 *main_out, float2 *ref_unclamped_out) {
+  launch_nesterov_move_ex(c, ref, grad, main_in, main_out, ref_unclamped_out, nullptr, nullptr);
+}
+
+void launch_done_check_ex(Ctx &c, int next_iter, double *log, int log_cap, int set_iter) {
+  hipLaunchKernelGGL(done_check_kernel, dim3(1), dim3(1), 0, c.stream, c.sc, next_iter, log, log_cap, set_iter);
+}
+
+void launch_done_check(Ctx &c, int next_iter) { launch_done_check_ex(c, next_iter, nullptr, 0, 1); }
+
+void launch_count_eval(Ctx &c) {
+  hipLaunchKernelGGL(count_eval_kernel, dim3(1), dim3(1), 0, c.stream, c.sc);
+}
+
+}  // namespace ep

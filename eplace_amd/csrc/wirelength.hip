@@ -1,0 +1,280 @@
+// Per-net bound pins, HPWL and the weighted-average (or log-sum-exp)
+// wirelength gradient, pins sorted by net.
+//
+// Reference: Net::calcBoundPin (objects.cpp:75-144),
+// Net::calcWirelengthWA_2D / getWirelengthGradientWA_2D (objects.cpp:248-415),
+// LSE (objects.cpp:161-241, 423-437), PlaceDB::calcHPWL (placedb.cpp:424-432).
+//
+// The WA gradient is evaluated in the max/min-shifted form
+//   g+ = e+_i/D+ (1 + ig (u_i - N'+/D+)),   u_i = x_i - x_max,  N'+ = sum u e+
+//   g- = e-_i/D- (1 - ig (v_i - N'-/D-)),   v_i = x_i - x_min,  N'- = sum v e-
+// which equals the reference's ((e + x ig e) D - ig e N)/D^2 form but does not
+// cancel catastrophically in float when |x| >> 1/ig.  Exponents are the
+// reference's (x - x_max) ig and (x_min - x) ig with its -300 cut-off
+// (global.h:46); exp_mode 0 evaluates (1 + a/1024)^1024 (global.h:457-471) in
+// double, exp_mode 1 uses expf.
+#include "ep_internal.h"
+#include "reduce.h"
+
+namespace ep {
+
+struct PinSrc {
+  const int32_t *net_start, *pin_obj;
+  const float *pdx, *pdy, *x, *y, *tx, *ty;
+  int N;
+  __device__ __forceinline__ float px(int p) const {
+    int o = pin_obj[p];
+    return (o < N ? x[o] : tx[o - N]) + pdx[p];
+  }
+  __device__ __forceinline__ float py(int p) const {
+    int o = pin_obj[p];
+    return (o < N ? y[o] : ty[o - N]) + pdy[p];
+  }
+};
+
+template <int EXPM>
+__device__ __forceinline__ float wl_exp(float a) {
+  if (a <= -300.f) return 0.f;
+  if (EXPM == 0) {
+    double b = 1.0 + (double)a * (1.0 / 1024.0);
+#pragma unroll
+    for (int i = 0; i < 10; ++i) b *= b;
+    return (float)b;
+  }
+  return expf(a);
+}
+
+struct NetAcc {
+  float dpx, npx, dmx, nmx, dpy, npy, dmy, nmy;
+};
+
+template <int MODEL, int EXPM>
+__device__ __forceinline__ void pin_terms(float x, float xmax, float xmin, float igx, float &ep, float &em) {
+  ep = wl_exp<EXPM>((x - xmax) * igx);
+  em = wl_exp<EXPM>((xmin - x) * igx);
+}
+
+// Pin gradient along one axis from the net sums.
+template <int MODEL>
+__device__ __forceinline__ float pin_grad_axis(float x, float xmax, float xmin, float ig, float ep, float em,
+                                               float dp, float np, float dm, float nm) {
+  if (MODEL == 1) return ep / dp - em / dm;  // LSE
+  float gp = 0.f, gm = 0.f;
+  if (ep != 0.f) gp = ep / dp * (1.f + ig * ((x - xmax) - np / dp));
+  if (em != 0.f) gm = em / dm * (1.f - ig * ((x - xmin) - nm / dm));
+  return gp - gm;
+}
+
+template <int MODEL>
+__device__ __forceinline__ double net_value(float xmax, float xmin, float ig, float dp, float np, float dm,
+                                            float nm) {
+  if (MODEL == 1)
+    return (double)(xmax - xmin) + (double)(logf(dp) / ig) + (double)(logf(dm) / ig);
+  return ((double)xmax + (double)(np / dp)) - ((double)xmin + (double)(nm / dm));
+}
+
+// One thread per net (degree <= kSmallNetMaxDegree), grid-stride.
+template <int MODEL, int EXPM, int GRAD>
+__global__ void __launch_bounds__(kBlock) wl_small_kernel(int num, const int32_t *__restrict__ nets,
+                                                           PinSrc src, const DevScalars *sc,
+                                                           float2 *__restrict__ pin_grad, double *partials,
+                                                           int slot0) {
+  if (sc->done) return;
+  float igx = (float)sc->inv_gamma_x, igy = (float)sc->inv_gamma_y;
+  double hp = 0, wv = 0;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < num; t += gridDim.x * blockDim.x) {
+    int e = nets ? nets[t] : t;
+    int s = src.net_start[e], n = src.net_start[e + 1];
+    if (s == n) continue;
+    float xmax = -INFINITY, xmin = INFINITY, ymax = -INFINITY, ymin = INFINITY;
+    for (int p = s; p < n; ++p) {
+      float px = src.px(p), py = src.py(p);
+      xmax = fmaxf(xmax, px);
+      xmin = fminf(xmin, px);
+      ymax = fmaxf(ymax, py);
+      ymin = fminf(ymin, py);
+    }
+    hp += ((double)xmax - (double)xmin) + ((double)ymax - (double)ymin);
+    if (!GRAD) continue;
+    NetAcc a = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int p = s; p < n; ++p) {
+      float px = src.px(p), py = src.py(p);
+      float ep, em, fp, fm;
+      pin_terms<MODEL, EXPM>(px, xmax, xmin, igx, ep, em);
+      pin_terms<MODEL, EXPM>(py, ymax, ymin, igy, fp, fm);
+      a.dpx += ep;
+      a.npx += (px - xmax) * ep;
+      a.dmx += em;
+      a.nmx += (px - xmin) * em;
+      a.dpy += fp;
+      a.npy += (py - ymax) * fp;
+      a.dmy += fm;
+      a.nmy += (py - ymin) * fm;
+    }
+    wv += net_value<MODEL>(xmax, xmin, igx, a.dpx, a.npx, a.dmx, a.nmx) +
+          net_value<MODEL>(ymax, ymin, igy, a.dpy, a.npy, a.dmy, a.nmy);
+    for (int p = s; p < n; ++p) {
+      float px = src.px(p), py = src.py(p);
+      float ep, em, fp, fm;
+      pin_terms<MODEL, EXPM>(px, xmax, xmin, igx, ep, em);
+      pin_terms<MODEL, EXPM>(py, ymax, ymin, igy, fp, fm);
+      float gx = pin_grad_axis<MODEL>(px, xmax, xmin, igx, ep, em, a.dpx, a.npx, a.dmx, a.nmx);
+      float gy = pin_grad_axis<MODEL>(py, ymax, ymin, igy, fp, fm, a.dpy, a.npy, a.dmy, a.nmy);
+      pin_grad[p] = make_float2(gx, gy);
+    }
+  }
+  hp = block_sum(hp);
+  wv = block_sum(wv);
+  if (threadIdx.x == 0) {
+    partials[kPartialWL + 2 * (slot0 + blockIdx.x)] = hp;
+    partials[kPartialWL + 2 * (slot0 + blockIdx.x) + 1] = wv;
+  }
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ float wave_min(float v) {
+  for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ float wave_fsum(float v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// One wave per large net, grid-stride over nets.
+template <int MODEL, int EXPM, int GRAD>
+__global__ void __launch_bounds__(kBlock) wl_large_kernel(int num, const int32_t *__restrict__ nets,
+                                                           PinSrc src, const DevScalars *sc,
+                                                           float2 *__restrict__ pin_grad, double *partials,
+                                                           int slot0) {
+  if (sc->done) return;
+  float igx = (float)sc->inv_gamma_x, igy = (float)sc->inv_gamma_y;
+  int lane = threadIdx.x & 63;
+  int wpb = blockDim.x >> 6;
+  double hp = 0, wv = 0;
+  for (int t = blockIdx.x * wpb + (threadIdx.x >> 6); t < num; t += gridDim.x * wpb) {
+    int e = nets[t];
+    int s = src.net_start[e], n = src.net_start[e + 1];
+    float xmax = -INFINITY, xmin = INFINITY, ymax = -INFINITY, ymin = INFINITY;
+    for (int p = s + lane; p < n; p += 64) {
+      float px = src.px(p), py = src.py(p);
+      xmax = fmaxf(xmax, px);
+      xmin = fminf(xmin, px);
+      ymax = fmaxf(ymax, py);
+      ymin = fminf(ymin, py);
+    }
+    xmax = wave_max(xmax);
+    xmin = wave_min(xmin);
+    ymax = wave_max(ymax);
+    ymin = wave_min(ymin);
+    if (lane == 0) hp += ((double)xmax - (double)xmin) + ((double)ymax - (double)ymin);
+    if (!GRAD) continue;
+    NetAcc a = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int p = s + lane; p < n; p += 64) {
+      float px = src.px(p), py = src.py(p);
+      float ep, em, fp, fm;
+      pin_terms<MODEL, EXPM>(px, xmax, xmin, igx, ep, em);
+      pin_terms<MODEL, EXPM>(py, ymax, ymin, igy, fp, fm);
+      a.dpx += ep;
+      a.npx += (px - xmax) * ep;
+      a.dmx += em;
+      a.nmx += (px - xmin) * em;
+      a.dpy += fp;
+      a.npy += (py - ymax) * fp;
+      a.dmy += fm;
+      a.nmy += (py - ymin) * fm;
+    }
+    a.dpx = wave_fsum(a.dpx);
+    a.npx = wave_fsum(a.npx);
+    a.dmx = wave_fsum(a.dmx);
+    a.nmx = wave_fsum(a.nmx);
+    a.dpy = wave_fsum(a.dpy);
+    a.npy = wave_fsum(a.npy);
+    a.dmy = wave_fsum(a.dmy);
+    a.nmy = wave_fsum(a.nmy);
+    if (lane == 0)
+      wv += net_value<MODEL>(xmax, xmin, igx, a.dpx, a.npx, a.dmx, a.nmx) +
+            net_value<MODEL>(ymax, ymin, igy, a.dpy, a.npy, a.dmy, a.nmy);
+    for (int p = s + lane; p < n; p += 64) {
+      float px = src.px(p), py = src.py(p);
+      float ep, em, fp, fm;
+      pin_terms<MODEL, EXPM>(px, xmax, xmin, igx, ep, em);
+      pin_terms<MODEL, EXPM>(py, ymax, ymin, igy, fp, fm);
+      float gx = pin_grad_axis<MODEL>(px, xmax, xmin, igx, ep, em, a.dpx, a.npx, a.dmx, a.nmx);
+      float gy = pin_grad_axis<MODEL>(py, ymax, ymin, igy, fp, fm, a.dpy, a.npy, a.dmy, a.nmy);
+      pin_grad[p] = make_float2(gx, gy);
+    }
+  }
+  hp = block_sum(hp);
+  wv = block_sum(wv);
+  if (threadIdx.x == 0) {
+    partials[kPartialWL + 2 * (slot0 + blockIdx.x)] = hp;
+    partials[kPartialWL + 2 * (slot0 + blockIdx.x) + 1] = wv;
+  }
+}
+
+// Sums the per-block HPWL/WL partials in a fixed order; optionally applies
+// updatePenaltyFactor (eplace.cpp:846-883) with FP64 HPWL and lambda.
+__global__ void __launch_bounds__(kBlock) reduce_wl_kernel(int nslots, double *partials, DevScalars *sc,
+                                                            int lambda_update) {
+  if (sc->done) return;
+  double hp = block_sum_partials_strided(partials + kPartialWL, nslots, 2);
+  double wv = block_sum_partials_strided(partials + kPartialWL + 1, nslots, 2);
+  if (threadIdx.x == 0) {
+    sc->hpwl = hp;
+    sc->wl_value = wv;
+    if (lambda_update) {
+      double delta = hp - sc->last_hpwl;
+      double mult = delta < 0.0 ? 1.05 : pow(1.05, -delta / 350000.0 + 1.0);
+      mult = fmin(1.05, fmax(0.95, mult));
+      sc->lambda *= mult;
+      sc->last_hpwl = hp;
+    }
+  }
+}
+
+static int g_small_blocks(const Ctx &c) {
+  int b = (c.num_small + kBlock - 1) / kBlock;
+  return b < 1 ? 1 : (b > 8192 ? 8192 : b);
+}
+static int g_large_blocks(const Ctx &c) {
+  int wpb = kBlock / 64;
+  int b = (c.num_large + wpb - 1) / wpb;
+  return b < 1 ? 1 : (b > 4096 ? 4096 : b);
+}
+
+template <int MODEL, int EXPM, int GRAD>
+static void launch_wl_t(Ctx &c) {
+  PinSrc src{c.net_start, c.pin_obj, c.pin_dx, c.pin_dy, c.x, c.y, c.tx, c.ty, c.N};
+  int bs = g_small_blocks(c);
+  hipLaunchKernelGGL((wl_small_kernel<MODEL, EXPM, GRAD>), dim3(bs), dim3(kBlock), 0, c.stream, c.num_small,
+                     c.small_nets, src, c.sc, c.pin_grad, c.partials, 0);
+  if (c.num_large) {
+    hipLaunchKernelGGL((wl_large_kernel<MODEL, EXPM, GRAD>), dim3(g_large_blocks(c)), dim3(kBlock), 0,
+                       c.stream, c.num_large, c.large_nets, src, c.sc, c.pin_grad, c.partials, bs);
+  }
+}
+
+void launch_wirelength(Ctx &c, bool gradient) {
+  if (!gradient) {
+    launch_wl_t<0, 0, 0>(c);
+    return;
+  }
+  if (c.wl_model == 0) {
+    if (c.exp_mode == 0) launch_wl_t<0, 0, 1>(c);
+    else launch_wl_t<0, 1, 1>(c);
+  } else {
+    if (c.exp_mode == 0) launch_wl_t<1, 0, 1>(c);
+    else launch_wl_t<1, 1, 1>(c);
+  }
+}
+
+void launch_reduce_wirelength(Ctx &c, int lambda_update) {
+  int n = g_small_blocks(c) + (c.num_large ? g_large_blocks(c) : 0);
+  hipLaunchKernelGGL(reduce_wl_kernel, dim3(1), dim3(kBlock), 0, c.stream, n, c.partials, c.sc, lambda_update);
+}
+
+}  // namespace ep

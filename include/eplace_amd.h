@@ -1,0 +1,176 @@
+/* eplace_amd.h - C ABI of the MI355X global-placement engine.
+ *
+ * Drop-in boundary for the reference ePlace placer's global-placement path
+ * (Xinyu-lumos/ePlace).  The reference has no FFI: its optimiser holds an
+ * EPlacer_2D* and calls C++ methods on it (Optimization/nesterov.hpp:59-415,
+ * EPlace/eplace.h:95-200).  Each entry point below replaces one of those
+ * methods and is cited next to it.  Plain pointers and sizes only; positions
+ * and gradients cross the boundary as interleaved (x, y) float pairs in the
+ * reference's stage order (eplace.cpp:190-203, 742-804).
+ *
+ * All functions return EP_OK (0) or an EP_ERR_* code; ep_last_error() then
+ * describes the failure.  A context is bound to one HIP device and one
+ * stream and is not thread-safe.
+ */
+#ifndef EPLACE_AMD_H
+#define EPLACE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ep_ctx ep_ctx;
+
+enum {
+  EP_OK = 0,
+  EP_ERR_ARG = 1,     /* bad argument or size */
+  EP_ERR_HIP = 2,     /* HIP runtime error */
+  EP_ERR_STATE = 3,   /* call not valid in this state */
+  EP_ERR_NUMERIC = 4, /* NaN/inf in a scalar (e.g. Lipschitz 0/0) */
+  EP_ERR_NOMEM = 5
+};
+
+/* PLACEMENT_STAGE (eplace.h:21-27). */
+enum { EP_STAGE_MGP = 0, EP_STAGE_FILLERONLY = 1, EP_STAGE_CGP = 2 };
+
+/* Design arrays; replaces PlaceDB's dbNodes/dbTerminals/dbPins/dbNets/
+ * dbSiteRows (PlaceDB/placedb.h:88-93).  Nodes in dbNodes order, terminals
+ * in dbTerminals order, pins sorted by net (parser order, parser.cpp:473-680),
+ * positions are centres, pin offsets are from the centre
+ * (objects.cpp:454-459).  pin_obj < num_nodes names a node, otherwise
+ * terminal pin_obj - num_nodes. */
+typedef struct ep_design {
+  int32_t num_nodes, num_terminals, num_nets, num_rows;
+  int64_t num_pins;
+  const float *node_w, *node_h, *node_x, *node_y;
+  const float *term_w, *term_h, *term_x, *term_y;
+  const uint8_t *term_ni;
+  const int64_t *net_start; /* [num_nets + 1] */
+  const int32_t *pin_obj;   /* [num_pins] */
+  const float *pin_dx, *pin_dy;
+  const double *row_bottom, *row_height, *row_x0, *row_x1;
+  double common_row_height;
+} ep_design;
+
+typedef struct ep_params {
+  float target_density;     /* EPlacer_2D::setTargetDensity (eplace.cpp:27-31) */
+  int32_t bins_x, bins_y;   /* 0: the reference's bin rule (eplace.cpp:237-261) */
+  int32_t wirelength_model; /* 0 WA (default), 1 LSE (-LSE, eplace.cpp:512-530) */
+  int32_t exp_mode;         /* 0 reference (1+a/1024)^1024 (global.h:457-471), 1 exp */
+  uint32_t filler_seed;     /* rand() seed for filler placement; 1 = reference */
+  const float *filler_xy;   /* optional filler centres, overrides the seed */
+  int32_t filler_count;     /* -1: reference count (eplace.cpp:167) */
+  int32_t poisson_mode;     /* 0 FP32 MFMA, 1 split-BF16 (3 products) MFMA */
+  int32_t device;           /* HIP device ordinal */
+} ep_params;
+
+typedef struct ep_info {
+  int64_t num_nodes, num_fillers, num_objects, num_terminals, num_nets, num_pins;
+  int32_t bins_x, bins_y, fixed_shift, num_big_objects, num_large_nets, pad;
+  double bin_step_x, bin_step_y, core_llx, core_lly, core_urx, core_ury;
+  double filler_w, filler_h, std_cell_area, macro_area, total_row_area;
+} ep_info;
+
+/* Scalars the reference keeps as EPlacer_2D fields (eplace.h:128-136). */
+typedef struct ep_stats {
+  double overflow;      /* globalDensityOverflow (tau) of the last evaluation */
+  double lambda;        /* penalty factor */
+  double last_hpwl;     /* lastHPWL */
+  double hpwl;          /* HPWL at the last evaluated positions */
+  double inv_gamma_x, inv_gamma_y;
+  double wirelength;    /* WA (or LSE) value of the last evaluation */
+  int32_t stage;
+  int32_t mgp_iterations;
+} ep_stats;
+
+/* Optimiser settings (nesterov.hpp:86-149, 231-415). */
+typedef struct ep_opt_params {
+  int32_t method;           /* 0 vanilla (default), 1 backtracking (-bktrk) */
+  float target_overflow;    /* mGP stop threshold (default 0.10) */
+  int32_t max_iterations;   /* MAX_ITERATION (1000); stop when iter > this */
+  int32_t max_backtracks;   /* cap on step retries per iteration (reference: none) */
+  int32_t sync_every;       /* host polls the stop flag every this many iterations */
+  int32_t log_capacity;     /* rows available in log (may be 0) */
+  double *log;              /* optional [log_capacity][8]: iter, tau, lambda, hpwl, step, invgx, evals, backtracks */
+} ep_opt_params;
+
+typedef struct ep_run_stats {
+  int32_t iterations;       /* accepted Nesterov iterations */
+  int32_t evaluations;      /* full gradient evaluations */
+  int32_t backtracks;
+  int32_t stopped;          /* 1 if the stop condition was met */
+  double final_overflow;
+  double final_hpwl;        /* HPWL after wrap_up (main solution written back) */
+  double seconds;           /* wall time of the loop */
+} ep_run_stats;
+
+/* Which device array ep_read/ep_write address (parity and debugging). */
+enum {
+  EP_ARR_POSITIONS = 0,     /* float [O][2] centres, all objects (nodes then fillers) */
+  EP_ARR_NODE_Q = 1,        /* int64 [nx*ny] fixed-point node density */
+  EP_ARR_FILLER_Q = 2,      /* int64 [nx*ny] fixed-point filler density */
+  EP_ARR_RHO = 3,           /* float [nx*ny] charge density */
+  EP_ARR_PHI = 4,           /* float [nx*ny] */
+  EP_ARR_EX = 5,            /* float [nx*ny] */
+  EP_ARR_EY = 6,            /* float [nx*ny] */
+  EP_ARR_GRAD_DENSITY = 7,  /* float [O][2] */
+  EP_ARR_GRAD_WL = 8,       /* float [N][2] */
+  EP_ARR_GRAD_TOTAL = 9,    /* float [O][2] preconditioned direction */
+  EP_ARR_TERM_MAP = 10,     /* float [nx*ny] */
+  EP_ARR_BASE_MAP = 11,     /* float [nx*ny] */
+  EP_ARR_PIN_GRAD = 12,     /* float [P][2] */
+  EP_ARR_MAIN = 13          /* float [O][2] optimiser main solution */
+};
+
+/* new EPlacer_2D(db) + setTargetDensity + initialization() minus its first
+ * gradient (eplace.cpp:39-50, 61-426): fillers, bin grid, terminal/base
+ * density, device buffers. */
+int ep_create(const ep_design *design, const ep_params *params, ep_ctx **out);
+int ep_destroy(ep_ctx *ctx);
+const char *ep_last_error(const ep_ctx *ctx);
+int ep_get_info(ep_ctx *ctx, ep_info *info);
+
+/* EPlacer_2D::totalGradientUpdate (eplace.cpp:687-738). */
+int ep_total_gradient_update(ep_ctx *ctx);
+/* EPlacer_2D::getPosition / setPosition / getGradient (eplace.cpp:742-804);
+ * n = number of objects of the current stage. */
+int ep_get_position(ep_ctx *ctx, float *xy, int64_t n);
+int ep_set_position(ep_ctx *ctx, const float *xy, int64_t n);
+int ep_get_gradient(ep_ctx *ctx, float *xy, int64_t n);
+int64_t ep_stage_size(ep_ctx *ctx);
+/* EPlacer_2D::penaltyFactorInitilization / updatePenaltyFactor
+ * (eplace.cpp:812-883). */
+int ep_penalty_factor_init(ep_ctx *ctx);
+int ep_update_penalty_factor(ep_ctx *ctx);
+/* EPlacer_2D::switch2FillerOnly / switch2cGP (eplace.cpp:887-904). */
+int ep_switch_to_filler_only(ep_ctx *ctx);
+int ep_switch_to_cgp(ep_ctx *ctx, int32_t mgp_iterations);
+int ep_get_stats(ep_ctx *ctx, ep_stats *stats);
+/* PlaceDB::calcHPWL (placedb.cpp:424-432) at the current positions. */
+int ep_hpwl(ep_ctx *ctx, double *hpwl);
+
+/* FirstOrderOptimizer<VECTOR_3D>::opt() with EplaceNesterovOpt
+ * (opt.hpp:29-36, nesterov.hpp:86-415), run device-resident: init, loop
+ * until the stage's stop condition, wrap_up. */
+int ep_nesterov_run(ep_ctx *ctx, const ep_opt_params *opt, ep_run_stats *out);
+
+/* FFT_2D::doFFT on a caller-supplied density (fft.cpp:190-284), on the
+ * context's grid: rho [nx*ny] in, phi/ex/ey [nx*ny] out (phi may be NULL). */
+int ep_poisson(ep_ctx *ctx, const float *rho, float *phi, float *ex, float *ey);
+
+/* Raw access to device arrays (EP_ARR_*); count is in elements of the
+ * array's element type (float pairs count as 2). */
+int ep_read(ep_ctx *ctx, int32_t which, void *host, int64_t count);
+int ep_write(ep_ctx *ctx, int32_t which, const void *host, int64_t count);
+
+/* Wall time in seconds of the last ep_total_gradient_update's kernels, split
+ * per phase: [scatter, density finalise, wirelength, poisson, gather+combine]. */
+int ep_last_timings(ep_ctx *ctx, double *seconds5);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EPLACE_AMD_H */

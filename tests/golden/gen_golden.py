@@ -1,0 +1,105 @@
+"""Generate golden fixtures from the reference placer (run in the container
+that has the reference tree; the fixtures are committed, the reference is not).
+
+    python tests/golden/gen_golden.py
+
+Builds oracle/_ref/ref_harness from the reference's own sources
+(oracle/Makefile.ref), runs it on the reference's smoke design and on seeded
+synthetic designs written as Bookshelf, and stores its dumps as .npz files
+(inputs and outputs only) next to this script.
+"""
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = "/root/reference"
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+sys.path.insert(0, ROOT)
+
+from eplace_amd.design import synthetic, write_bookshelf  # noqa: E402
+
+# Keys kept per gp run (everything else the harness writes is dropped).
+KEEP = ["nodes_whxy", "nodes_macro", "terms_whxyll", "terms_ni", "pin_obj", "pin_offset", "net_start", "rows",
+        "fillers_init", "fillers_wh", "init_ll", "init_positions", "init_map_node", "init_map_filler",
+        "init_map_terminal", "init_map_base", "init_map_ex", "init_map_ey", "init_map_phi", "init_grad_density",
+        "init_grad_wirelength", "init_grad_total", "it0_main", "it1_main", "it2_main", "it0_db_positions",
+        "it1_db_positions", "it2_db_positions", "trajectory", "final_positions", "legal_positions"]
+
+
+def run(args, out):
+    if os.path.exists(out):
+        shutil.rmtree(out)
+    with open(os.path.join(tempfile.gettempdir(), "ref_harness.log"), "w") as log:
+        subprocess.run([HARNESS] + args + ["-out", out], check=True, stdout=log, stderr=log,
+                       cwd=tempfile.gettempdir())
+
+
+def pack(name, out, extra=None):
+    arrays = {}
+    for k in KEEP:
+        f = os.path.join(out, k + ".npy")
+        if os.path.exists(f):
+            arrays[k] = np.load(f)
+    scal = {}
+    for line in open(os.path.join(out, "scalars.txt")):
+        k, v = line.split()
+        scal[k] = float(v)
+    arrays["scalar_names"] = np.array(sorted(scal))
+    arrays["scalar_values"] = np.array([scal[k] for k in sorted(scal)])
+    for k, v in (extra or {}).items():
+        arrays[k] = np.asarray(v)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+    print(f"{name}: {len(arrays)} arrays, iterations {scal.get('iterations')}, final HPWL {scal.get('final_hpwl')}")
+
+
+def main():
+    subprocess.run(["make", "-s", "-j8", "-f", "oracle/Makefile.ref", f"REF={REF}"], cwd=ROOT, check=True)
+    tmp = tempfile.mkdtemp(prefix="golden_")
+    smoke = os.path.join(REF, "examples", "smoke", "smoke.aux")
+    # 1. the reference's own smoke flow (QP -> mGP -> Abacus), default and -bktrk
+    run(["gp", "-aux", smoke, "-targetDensity", "0.8", "-targetOverflow", "0.1", "-legal"], tmp + "/s")
+    pack("smoke_default", tmp + "/s")
+    run(["gp", "-aux", smoke, "-targetDensity", "0.8", "-targetOverflow", "0.1", "-bktrk"], tmp + "/sb")
+    pack("smoke_bktrk", tmp + "/sb")
+    run(["gp", "-aux", smoke, "-targetDensity", "0.8", "-targetOverflow", "0.1", "-LSE", "-iters", "3"],
+        tmp + "/sl")
+    pack("smoke_lse", tmp + "/sl")
+    # 2. seeded synthetic designs, no QP (the .pl is the start for both sides)
+    designs = {
+        "synth_c1": dict(num_cells=100, num_nets=184, seed=11, degrees="c1", widths="uniform14", td=0.8),
+        "synth_c2": dict(num_cells=2000, num_nets=2100, seed=12, td=1.0),
+        "synth_mixed": dict(num_cells=2000, num_nets=2200, seed=13, num_macros=4, util=0.75, td=1.0),
+    }
+    for name, kw in designs.items():
+        td = kw.pop("td")
+        d = synthetic(**kw, name=name)
+        aux = write_bookshelf(d, os.path.join(tmp, name))
+        run(["gp", "-aux", aux, "-targetDensity", str(td), "-targetOverflow", "0.1", "-noQP",
+             "-loadpl", aux[:-4] + ".pl", "-dumpEvery", "1000"], tmp + "/out_" + name)
+        pack(name, tmp + "/out_" + name, {"target_density": td})
+    # 3. the Poisson solve alone on random densities (square and non-square bins)
+    rng = np.random.default_rng(5)
+    fft = {}
+    for (nx, ny, bx, by) in [(16, 16, 1.25, 1.25), (32, 64, 1.5, 2.0), (128, 128, 0.8, 0.8)]:
+        rho = (rng.random((nx, ny)) * 1.5).astype(np.float32)
+        f = os.path.join(tmp, "rho.f32")
+        rho.tofile(f)
+        run(["fft", "-nx", str(nx), "-ny", str(ny), "-bx", str(bx), "-by", str(by), "-in", f], tmp + "/fft")
+        key = f"{nx}x{ny}"
+        fft[key + "_rho"] = rho
+        fft[key + "_bins"] = np.array([bx, by], np.float32)
+        for k in ("ex", "ey", "phi"):
+            fft[key + "_" + k] = np.load(os.path.join(tmp, "fft", k + ".npy"))
+    np.savez_compressed(os.path.join(HERE, "poisson.npz"), **fft)
+    print("poisson: ", sorted(fft))
+    shutil.rmtree(tmp)
+
+
+if __name__ == "__main__":
+    main()

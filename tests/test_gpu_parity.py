@@ -1,0 +1,233 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Bars: the fixed-point density maps are bit-exact; fields, potential and
+gradients within 1e-5 relative (norm-wise) of the FP64 oracle evaluated on
+the same inputs.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from eplace_amd import EPlacer2D, EplaceNesterovOpt, synthetic
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    n = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (n if n > 0 else 1.0))
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+CASES = {
+    "c1_16": dict(kw=dict(num_cells=100, num_nets=184, seed=1, degrees="c1", widths="uniform14"), td=0.8, bins=16),
+    "c2_small_64": dict(kw=dict(num_cells=3000, num_nets=3200, seed=2), td=1.0, bins=64),
+    "macro_64": dict(kw=dict(num_cells=3000, num_nets=3300, seed=4, num_macros=6, util=0.75), td=1.0, bins=64),
+    "rect_128x64": dict(kw=dict(num_cells=5000, num_nets=5200, seed=5), td=0.9, bins=(128, 64)),
+}
+
+
+def make(case, exp="reference", wl="WA", mode="fp64"):
+    c = CASES[case]
+    d = synthetic(**c["kw"], name=case)
+    gp = EPlacer2D(d, c["td"], bins=c["bins"], exp=exp, wirelength=wl)
+    orc = Oracle(d, c["td"], mode=mode, bins=c["bins"], exp="fast" if exp == "reference" else "exp", wl=wl)
+    # The device state is centres; lower-left corners are derived as
+    # c - w/2.  The oracle keeps the corner a filler was placed at
+    # (Module::coor), which can differ by one ulp; re-derive it so both
+    # sides scatter identical rectangles.
+    orc.set("positions", orc.get("positions"))
+    return d, gp, orc
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_density_fixed_point_bit_exact(case):
+    say(f"[density] case={case}")
+    d, gp, orc = make(case)
+    np.testing.assert_array_equal(gp.read("positions").reshape(-1), orc.get("positions").astype(np.float32))
+    np.testing.assert_array_equal(gp.read("term_map").reshape(-1), orc.get("term_map").astype(np.float32))
+    np.testing.assert_array_equal(gp.read("base_map").reshape(-1), orc.get("base_map").astype(np.float32))
+    gp.totalGradientUpdate()
+    shift = gp.info()["fixed_shift"]
+    orc.run("scatter_fixed", shift)
+    np.testing.assert_array_equal(gp.read("node_q").reshape(-1), orc.get("node_q"))
+    np.testing.assert_array_equal(gp.read("filler_q").reshape(-1), orc.get("filler_q"))
+    orc.run("overflow")
+    st = gp.stats()
+    # per-bin terms are float on the GPU, double in the oracle; summed in FP64
+    assert abs(st["overflow"] - orc.info()["overflow"]) <= 1e-6 * max(1.0, abs(orc.info()["overflow"]))
+    orc.run("rho")
+    assert rel(gp.read("rho"), orc.map("rho")) < 1e-6
+
+
+@pytest.mark.parametrize("n", [(16, 16), (64, 64), (128, 64), (256, 256), (512, 512)])
+def test_poisson_vs_fp64(n):
+    say(f"[poisson] grid={n}")
+    d = synthetic(200, 200, seed=9, name="p")
+    gp = EPlacer2D(d, 1.0, bins=n)
+    i = gp.info()
+    rng = np.random.default_rng(7)
+    rho = rng.random(n).astype(np.float32) * 2.0
+    phi, ex, ey = gp.poisson(rho)
+    from oracle.oracle import poisson_f64
+    rphi, rex, rey = poisson_f64(rho.astype(np.float64), i["bin_step_x"], i["bin_step_y"], "fast")
+    for a, b, nm in ((ex, rex, "ex"), (ey, rey, "ey"), (phi, rphi, "phi")):
+        e = rel(a, b)
+        say(f"  {nm} rel err {e:.2e}")
+        assert e < 1e-5, (nm, e)
+
+
+@pytest.mark.parametrize("case", list(CASES))
+@pytest.mark.parametrize("exp", ["reference", "exp"])
+def test_wirelength_gradient(case, exp):
+    say(f"[wirelength] case={case} exp={exp}")
+    d, gp, orc = make(case, exp=exp)
+    gp.totalGradientUpdate()
+    st = gp.stats()
+    orc.set_scalar("invgx", st["inv_gamma_x"])
+    orc.set_scalar("invgy", st["inv_gamma_y"])
+    orc.run("wirelength")
+    gw = gp.read("grad_wirelength")
+    e = rel(gw, orc.get("grad_wirelength").reshape(-1, 2))
+    say(f"  grad_wl rel err {e:.2e}")
+    assert e < 1e-5
+    assert abs(st["hpwl"] - orc.info()["hpwl_bound"]) <= 1e-9 * orc.info()["hpwl_bound"]
+    assert abs(st["wirelength"] - orc.info()["wl_value"]) <= 1e-5 * orc.info()["wl_value"]
+
+
+@pytest.mark.parametrize("case", ["c2_small_64", "macro_64"])
+def test_lse_gradient(case):
+    say(f"[lse] case={case}")
+    d, gp, orc = make(case, wl="LSE")
+    gp.totalGradientUpdate()
+    st = gp.stats()
+    orc.set_scalar("invgx", st["inv_gamma_x"])
+    orc.set_scalar("invgy", st["inv_gamma_y"])
+    orc.run("wirelength")
+    e = rel(gp.read("grad_wirelength"), orc.get("grad_wirelength").reshape(-1, 2))
+    say(f"  grad_lse rel err {e:.2e}")
+    assert e < 1e-5
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_gather_and_combine(case):
+    say(f"[gather] case={case}")
+    d, gp, orc = make(case)
+    gp.totalGradientUpdate()
+    gp.penaltyFactorInitilization()
+    gp.totalGradientUpdate()
+    st = gp.stats()
+    # same field and wirelength gradient on both sides, then gather + combine
+    orc.set("ex", gp.read("ex").reshape(-1).astype(np.float64))
+    orc.set("ey", gp.read("ey").reshape(-1).astype(np.float64))
+    orc.run("gather")
+    e = rel(gp.read("grad_density"), orc.get("grad_density").reshape(-1, 2))
+    say(f"  grad_density rel err {e:.2e}")
+    assert e < 1e-5
+    orc.set("grad_wirelength", gp.read("grad_wirelength").reshape(-1).astype(np.float64))
+    orc.set("grad_density", gp.read("grad_density").reshape(-1).astype(np.float64))
+    orc.set_scalar("lambda", st["lambda"])
+    orc.run("combine")
+    e = rel(gp.read("grad_total"), orc.get("grad_total").reshape(-1, 2))
+    say(f"  grad_total rel err {e:.2e}")
+    assert e < 1e-6
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_total_gradient_end_to_end(case):
+    say(f"[total] case={case}")
+    d, gp, orc = make(case)
+    gp.totalGradientUpdate()
+    shift = gp.info()["fixed_shift"]
+    orc.run("total_gradient_update", 1, shift)
+    for nm in ("ex", "ey"):
+        e = rel(gp.read(nm), orc.map(nm))
+        say(f"  {nm} rel err {e:.2e}")
+        assert e < 1e-5
+    e = rel(gp.read("grad_density"), orc.get("grad_density").reshape(-1, 2))
+    say(f"  grad_density rel err {e:.2e}")
+    assert e < 1e-5
+    e = rel(gp.read("grad_wirelength"), orc.get("grad_wirelength").reshape(-1, 2))
+    say(f"  grad_wl rel err {e:.2e}")
+    assert e < 1e-5
+
+
+def test_lambda0_and_penalty_update():
+    say("[lambda]")
+    d, gp, orc = make("c2_small_64")
+    gp.initialization()
+    orc.run("total_gradient_update", 1, gp.info()["fixed_shift"])
+    orc.run("penalty_init")
+    st = gp.stats()
+    assert abs(st["lambda"] - orc.info()["lambda"]) < 1e-5 * orc.info()["lambda"]
+    assert abs(st["last_hpwl"] - orc.info()["last_hpwl"]) < 1e-9 * orc.info()["last_hpwl"]
+    # move everything a little and update lambda on both sides
+    xy = gp.getPosition() + np.float32(0.37)
+    gp.setPosition(xy)
+    orc.set("stage_positions_clamped", xy.reshape(-1).astype(np.float64))
+    np.testing.assert_array_equal(gp.read("positions").reshape(-1), orc.get("positions").astype(np.float32))
+    gp.updatePenaltyFactor()
+    orc.run("update_penalty")
+    st = gp.stats()
+    assert abs(st["lambda"] - orc.info()["lambda"]) < 1e-5 * orc.info()["lambda"]
+
+
+# Final-HPWL bar per design: 0.5% (the target) where the design is large
+# enough for the run-to-run spread of the algorithm itself to be well below
+# it; on the 100-cell design the reference-float and FP64 oracle runs of the
+# same algorithm already differ by 1.2%, so 2% there.
+@pytest.mark.parametrize("case,tol", [("c2_small_64", 0.005), ("c1_16", 0.02)])
+@pytest.mark.parametrize("method", ["vanilla", "bktrk"])
+def test_nesterov_run_matches_oracle(method, case, tol):
+    say(f"[nesterov] method={method} case={case}")
+    d, gp, orc = make(case)
+    gp.initialization()
+    shift = gp.info()["fixed_shift"]
+    orc.run("total_gradient_update", 1, shift)
+    orc.run("penalty_init")
+    orc.run("opt_init")
+    k = 0
+    while not orc.stop_condition(0.1):
+        orc.run("opt_step", 1 if method == "bktrk" else 0, shift)
+        k += 1
+    orc.run("wrap_up")
+    r = EplaceNesterovOpt(gp, method=method).opt()
+    say(f"  gpu iters {r['iterations']} hpwl {r['final_hpwl']:.3f}; oracle iters {k} hpwl {orc.hpwl():.3f}")
+    assert abs(r["iterations"] - k) <= max(5, 0.05 * k)
+    assert abs(r["final_hpwl"] - orc.hpwl()) <= tol * orc.hpwl()
+
+
+@pytest.mark.parametrize("case", ["c1_16", "c2_small_64"])
+def test_vanilla_trajectory_first_iterations(case):
+    """Step-by-step agreement before the runs separate (chaotic dynamics)."""
+    say(f"[trajectory] case={case}")
+    d, gp, orc = make(case)
+    gp.initialization()
+    shift = gp.info()["fixed_shift"]
+    orc.run("total_gradient_update", 1, shift)
+    orc.run("penalty_init")
+    orc.run("opt_init")
+    K = 12
+    otraj = []
+    for k in range(K):
+        orc.run("opt_step", 0, shift)
+        i = orc.info()
+        otraj.append((i["overflow"], i["lambda"], i["last_hpwl"], i["last_step"]))
+    opt = EplaceNesterovOpt(gp, max_iterations=K - 1)
+    r = opt.opt()
+    log = opt.log
+    for k in range(min(K, len(log))):
+        gl = log[k]
+        say(f"  k={k:2d} tau gpu {gl[1]:.6f} orc {otraj[k][0]:.6f} | step gpu {gl[4]:.5g} orc {otraj[k][3]:.5g}"
+            f" | lam gpu {gl[2]:.5g} | hpwl(y_k) gpu {gl[3]:.6g}")
+    for k in range(4):
+        assert abs(log[k][1] - otraj[k][0]) <= 1e-3 * otraj[k][0]
+        assert abs(log[k][4] - otraj[k][3]) <= 2e-2 * otraj[k][3]

@@ -1,0 +1,124 @@
+"""The CPU oracle against the reference's own outputs (committed fixtures).
+
+In reference-float mode the oracle must reproduce the reference bit for bit
+wherever the reference's Ooura FFT is not involved: filler placement (glibc
+rand), bin grid, terminal/base/node/filler density maps, overflow, 1/gamma,
+the WA gradient and the gather given the same field.  Where the FFT is
+involved the bar is the reference's own float rounding (~1e-6).
+"""
+import numpy as np
+import pytest
+
+from golden_utils import Golden, rel
+from oracle.oracle import Oracle, poisson_f64
+
+DESIGNS = ["smoke_default", "synth_c1", "synth_c2", "synth_mixed"]
+
+
+def oracle_at_init(g, mode="ref", wl="WA"):
+    d = g.design()
+    o = Oracle(d, g.target_density, mode=mode, wl=wl)
+    o.set("ll", g["init_ll"].astype(np.float64))  # the reference's stored corners
+    return d, o
+
+
+@pytest.mark.parametrize("name", DESIGNS)
+def test_fillers_and_bins_bit_exact(name):
+    g = Golden(name)
+    d, o = oracle_at_init(g)
+    i = o.info()
+    assert i["F"] == int(g.s["num_fillers"])
+    assert (i["nx"], i["ny"]) == (int(g.s["bins_x"]), int(g.s["bins_y"]))
+    assert np.float32(i["bsx"]) == np.float32(g.s["bin_step_x"])
+    pos = o.get("positions").reshape(-1, 2).astype(np.float32)
+    np.testing.assert_array_equal(pos[i["N"]:], g["fillers_init"])
+    np.testing.assert_array_equal(o.get("wh").reshape(-1, 2)[i["N"]:].astype(np.float32), g["fillers_wh"])
+    np.testing.assert_array_equal(o.map("term_map").astype(np.float32), g["init_map_terminal"])
+    np.testing.assert_array_equal(o.map("base_map").astype(np.float32), g["init_map_base"])
+
+
+@pytest.mark.parametrize("name", DESIGNS)
+def test_first_evaluation_bit_exact(name):
+    g = Golden(name)
+    d, o = oracle_at_init(g)
+    o.run("total_gradient_update", 0, -1)
+    np.testing.assert_array_equal(o.map("node_map").astype(np.float32), g["init_map_node"])
+    np.testing.assert_array_equal(o.map("filler_map").astype(np.float32), g["init_map_filler"])
+    i = o.info()
+    assert np.float32(i["overflow"]) == np.float32(g.s["init_overflow"])
+    assert np.float32(i["invgx"]) == np.float32(g.s["init_inv_gamma_x"])
+    assert np.float32(i["invgy"]) == np.float32(g.s["init_inv_gamma_y"])
+    np.testing.assert_array_equal(o.get("grad_wirelength").reshape(-1, 2).astype(np.float32),
+                                  g["init_grad_wirelength"])
+    # field: the reference's float Ooura FFT vs the oracle's float FFT form
+    for k in ("ex", "ey", "phi"):
+        assert rel(o.map(k), g["init_map_" + k]) < 2e-6, k
+    # gather with the reference's own field is exact
+    o.set("ex", g["init_map_ex"].reshape(-1).astype(np.float64))
+    o.set("ey", g["init_map_ey"].reshape(-1).astype(np.float64))
+    o.run("gather")
+    np.testing.assert_array_equal(o.get("grad_density").reshape(-1, 2).astype(np.float32), g["init_grad_density"])
+    # combine at lambda = 0 (the first evaluation precedes lambda0)
+    o.run("combine")
+    np.testing.assert_array_equal(o.get("grad_total").reshape(-1, 2).astype(np.float32), g["init_grad_total"])
+    o.run("penalty_init")
+    assert o.info()["last_hpwl"] == g.s["hpwl0"]
+    np.testing.assert_allclose(o.info()["lambda"], g.s["lambda0"], rtol=1e-6)
+
+
+def test_lse_gradient_bit_exact():
+    g = Golden("smoke_lse")
+    d, o = oracle_at_init(g, wl="LSE")
+    o.run("total_gradient_update", 0, -1)
+    np.testing.assert_array_equal(o.get("grad_wirelength").reshape(-1, 2).astype(np.float32),
+                                  g["init_grad_wirelength"])
+
+
+@pytest.mark.parametrize("name", DESIGNS)
+def test_first_iterations(name):
+    g = Golden(name)
+    d, o = oracle_at_init(g)
+    o.run("total_gradient_update", 0, -1)
+    o.run("penalty_init")
+    o.run("opt_init")
+    for k in range(3):
+        o.run("opt_step", 0, -1)
+        m = o.get("cur_main").reshape(-1, 2)
+        ref = g[f"it{k}_main"]
+        scale = np.abs(ref).max()
+        # the FFT rounding difference (~1e-6) is amplified each step
+        assert np.abs(m - ref).max() <= 1e-6 * 10 ** k * scale, (k, np.abs(m - ref).max())
+
+
+# The 0.5% final-HPWL bar holds on the 2000-cell designs.  On the 100-cell
+# smoke design the run-to-run spread of the algorithm itself is ~1%: the
+# oracle's reference-float / FP64 / fixed-point variants of the -bktrk run end
+# between 240.8 and 243.0 (reference 240.68), so 1.5% there.
+@pytest.mark.parametrize("name,bktrk,tol", [("smoke_default", 0, 0.015), ("smoke_bktrk", 1, 0.015),
+                                            ("synth_c2", 0, 0.005), ("synth_mixed", 0, 0.005)])
+def test_full_run_matches_reference(name, bktrk, tol):
+    g = Golden(name)
+    d, o = oracle_at_init(g)
+    o.run("total_gradient_update", 0, -1)
+    o.run("penalty_init")
+    o.run("opt_init")
+    k = 0
+    while not o.stop_condition(0.1):
+        o.run("opt_step", bktrk, -1)
+        k += 1
+    o.run("wrap_up")
+    iters = int(g.s["iterations"])
+    assert abs(k - iters) <= max(3, 0.05 * iters), (k, iters)
+    assert abs(o.hpwl() - g.s["final_hpwl"]) <= tol * g.s["final_hpwl"], (o.hpwl(), g.s["final_hpwl"])
+
+
+@pytest.mark.parametrize("key", ["16x16", "32x64", "128x128"])
+@pytest.mark.parametrize("method", ["fast", "direct"])
+def test_poisson_closed_form_vs_reference_fft(key, method):
+    z = np.load(__import__("golden_utils").GOLDEN + "/poisson.npz")
+    rho = z[key + "_rho"].astype(np.float64)
+    bx, by = z[key + "_bins"]
+    phi, ex, ey = poisson_f64(rho, float(bx), float(by), method)
+    assert rel(ex, z[key + "_ex"]) < 2e-6
+    assert rel(ey, z[key + "_ey"]) < 2e-6
+    assert rel(phi, z[key + "_phi"]) < 2e-6
