@@ -1,0 +1,208 @@
+#!/usr/bin/env python3
+"""Global-placement benchmark of the MI355X engine.
+
+Workload: a seeded synthetic 1M-cell design in the std-cell recipe (widths
+lognormal median 3 sites, unit height, 1K boundary IO pins, 1.05M nets with
+degree 2:55% 3:20% 4-10:20% 11-200 power-law, index-local pins, utilisation
+0.7, target density 1.0; the reference's bin rule gives 1024x1024 bins).
+A step is one accepted Nesterov iteration of global placement (the plain
+backtracking method of nesterov.hpp:310-369 by default) with every state
+array resident in HBM.  `value` = K timed steps / wall time.  The same run
+also reports the wall time from the first gradient evaluation to overflow
+<= 0.10 and the CPU baseline: the oracle's restatement of the reference path
+(reference float arithmetic, one thread) on a bounded sample of the same
+design.
+
+    python bench.py [--steps K] [--warmup W] [--method bktrk|vanilla]
+"""
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
+FP32_MFMA_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix
+
+
+def device_sync():
+    ctypes.CDLL("libamdhip64.so").hipDeviceSynchronize()
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def roofline(phase, info, design):
+    """Algorithmic work per launch-group of the dominant phase."""
+    O, N, P, E = info["num_objects"], info["num_nodes"], info["num_pins"], info["num_nets"]
+    X = -(-info["bins_x"] // 64) * 64
+    Y = -(-info["bins_y"] // 64) * 64
+    nb = info["bins_x"] * info["bins_y"]
+    sec = phase["seconds"]
+    name = phase["name"]
+    if name == "poisson":
+        # 4 GEMM launches: rho F_Y^T, F_X T1, phi^ [C_Y|W_Y S_Y], batched Ex/Ey
+        flops = 2.0 * X * Y * Y + 2.0 * X * X * Y + 2.0 * X * Y * (2 * Y) + 2 * (2.0 * X * X * Y)
+        ach = flops / sec / 1e12
+        return {"bound": "mfma", "achieved": ach, "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": ach / FP32_MFMA_PEAK_TFLOPS, "traffic": None,
+                "kernel": "gemm_f32_kernel x4", "work_per_launch": flops / 4}
+    if name == "wirelength":
+        # per pin: obj idx, dx, dy, gathered centre (20 B) + pin gradient (8 B); per net: start + index
+        byts = 28.0 * P + 8.0 * E
+    elif name == "scatter":
+        # per object: x, y, w, h, kind, big flag; per bin: clear and accumulate two int64 maps
+        byts = 18.0 * O + 32.0 * nb
+    elif name == "finalize":
+        # per bin: two int64 maps, terminal and base maps in, rho out
+        byts = 28.0 * nb
+    else:
+        # per object: geometry, kind, pins count, area (26 B), previous y and g (16 B), outputs (24 B);
+        # per pin: CSR index + pin gradient (12 B); the two field maps once
+        byts = 66.0 * O + 12.0 * P + 8.0 * nb
+    ach = byts / sec / 1e9
+    return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "traffic": None, "kernel": name, "work_per_launch": byts}
+
+
+def cpu_baseline(design, method, iters, init_positions):
+    from oracle.oracle import Oracle
+    o = Oracle(design, 1.0, mode="ref")
+    o.set("positions", init_positions.reshape(-1).astype(np.float64))
+    o.run("total_gradient_update", 0, -1)
+    o.run("penalty_init")
+    o.run("opt_init")
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        o.run("opt_step", 1 if method == "bktrk" else 0, -1)
+    dt = time.perf_counter() - t0
+    return {"value": iters / dt, "unit": "iters/s", "cores": 1, "kind": "port",
+            "sample": f"{iters} {method} Nesterov iterations of the same {design.num_nodes}-cell design after "
+                      f"initialisation, oracle restatement of the reference path in reference float arithmetic, "
+                      f"1 thread on {cpu_model()}",
+            "seconds": dt}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cells", type=int, default=1_000_000)
+    ap.add_argument("--method", default="bktrk", choices=["bktrk", "vanilla"])
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--no-converge", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this engine runs global placement on one GPU; multi-GPU runs are not built "
+              "(see DESIGN.md for how the path would shard).", file=sys.stderr)
+        sys.exit(2)
+
+    from eplace_amd import EPlacer2D, EplaceNesterovOpt, synthetic
+
+    cells = args.cells
+    nets = int(round(cells * 1.05))
+    t = time.perf_counter()
+    d = synthetic(cells, nets, seed=args.seed, num_io=1000, util=0.7, name=f"synth_{cells}")
+    t_gen = time.perf_counter() - t
+    print(f"[bench] design {cells} cells, {nets} nets, {d.num_pins} pins (generated in {t_gen:.1f}s)",
+          file=sys.stderr, flush=True)
+    t = time.perf_counter()
+    gp = EPlacer2D(d, 1.0)
+    info = gp.info()
+    t_create = time.perf_counter() - t
+    print(f"[bench] engine ready in {t_create:.1f}s: {info['num_objects']} objects "
+          f"({info['num_fillers']} fillers), {info['bins_x']}x{info['bins_y']} bins", file=sys.stderr, flush=True)
+    init_pos = gp.read("positions").copy()
+
+    def restart():
+        gp.write("positions", init_pos)
+        gp.initialization()
+
+    # warmup: W iterations from the initial placement (discarded)
+    gp.initialization()
+    EplaceNesterovOpt(gp, method=args.method, max_iterations=max(args.warmup - 1, 0)).opt()
+    print("[bench] warmup done", file=sys.stderr, flush=True)
+
+    # timed: exactly K iterations from the initial placement
+    restart()
+    device_sync()
+    t0 = time.perf_counter()
+    opt = EplaceNesterovOpt(gp, method=args.method, max_iterations=args.steps - 1)
+    r = opt.opt(profile=True)
+    device_sync()
+    dt = time.perf_counter() - t0
+    steps = r["iterations"]
+    print(f"[bench] timed {steps} iterations in {dt:.3f}s, {r['evaluations']} evaluations, "
+          f"{r['backtracks']} backtracks", file=sys.stderr, flush=True)
+    ph = r["phase_seconds"]
+    dominant = max(ph, key=ph.get)
+    roof = roofline({"name": dominant, "seconds": ph[dominant]}, info, d)
+
+    conv = None
+    if not args.no_converge:
+        restart_t0 = time.perf_counter()
+        gp.write("positions", init_pos)
+        device_sync()
+        t0 = time.perf_counter()
+        gp.initialization()
+        rc = EplaceNesterovOpt(gp, method=args.method, target_overflow=0.10).opt()
+        device_sync()
+        tc = time.perf_counter() - t0
+        conv = {"seconds_to_overflow_0.10": tc, "iterations": rc["iterations"], "evaluations": rc["evaluations"],
+                "backtracks": rc["backtracks"], "final_overflow": rc["final_overflow"],
+                "final_hpwl": rc["final_hpwl"], "iters_per_s": rc["iterations"] / tc}
+        print(f"[bench] converged: {rc['iterations']} iterations in {tc:.3f}s, overflow {rc['final_overflow']:.4f}, "
+              f"HPWL {rc['final_hpwl']:.6g}", file=sys.stderr, flush=True)
+        del restart_t0
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(d, args.method, args.cpu_iters, init_pos)
+        print(f"[bench] cpu baseline {cpu['value']:.4f} iters/s over {cpu['seconds']:.1f}s", file=sys.stderr,
+              flush=True)
+
+    out = {
+        "metric": "GP time to overflow<=0.10 @1M cells, learned vs plain (HPWL +-0.5%); iters/s",
+        "value": steps / dt,
+        "unit": "iters/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"synthetic std-cell design, {cells} cells, {nets} nets, {d.num_pins} pins, "
+                               f"{info['num_fillers']} fillers, {info['bins_x']}x{info['bins_y']} bins, "
+                               f"util 0.7, target density 1.0, seed {args.seed}",
+                   "method": f"plain {args.method} Nesterov (device-resident)", "cells": cells,
+                   "bins": [info["bins_x"], info["bins_y"]]},
+        "roofline": roof,
+        "phase_ms_per_evaluation": {k: v * 1e3 for k, v in ph.items()},
+        "converge": conv,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -68,7 +68,7 @@ class EPOptParams(C.Structure):
     _fields_ = [
         ("method", C.c_int32), ("target_overflow", C.c_float), ("max_iterations", C.c_int32),
         ("max_backtracks", C.c_int32), ("sync_every", C.c_int32), ("log_capacity", C.c_int32),
-        ("log", C.c_void_p),
+        ("log", C.c_void_p), ("phase_seconds", C.c_void_p),
     ]
 
 

@@ -124,14 +124,12 @@ __global__ void __launch_bounds__(kBlock) scatter_big_kernel(const int32_t *__re
   }
 }
 
-// Fixed-point maps -> float maps -> rho (eplace.cpp:571-580) and the overflow
-// sum (eplace.cpp:438-453); the last block finishes tau and the gamma
-// schedule (eplace.cpp:475-500).
+// Fixed-point maps -> float maps -> rho (eplace.cpp:571-580) and per-block
+// partial sums of the overflow area (eplace.cpp:438-453) and of rho.
 __global__ void __launch_bounds__(kBlock) finalize_density_kernel(
     int nb, const unsigned long long *__restrict__ node_q, const unsigned long long *__restrict__ filler_q,
     const float *__restrict__ term, const float *__restrict__ base, Grid g, int ldp, double inv_scale,
-    double node_area_scaled, float *__restrict__ rho, double *partials, unsigned int *ticket,
-    DevScalars *sc) {
+    float *__restrict__ rho, double *partials, const DevScalars *sc) {
   if (sc->done) return;
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   double ovf = 0, rsum = 0;
@@ -153,31 +151,35 @@ __global__ void __launch_bounds__(kBlock) finalize_density_kernel(
     partials[kPartialTau + blockIdx.x] = s;
     partials[kPartialRho + blockIdx.x] = rs;
   }
-  if (last_block_done(ticket)) {
-    double tot = block_sum_partials(partials + kPartialTau, gridDim.x);
-    double rtot = block_sum_partials(partials + kPartialRho, gridDim.x);
-    if (threadIdx.x == 0) {
-      double tau = tot / node_area_scaled;
-      sc->overflow = tau;
-      sc->rho_mean = rtot / (double)nb;
-      // 1/gamma in the reference's float arithmetic
-      float cx = 0.125f / g.bsx, cy = 0.125f / g.bsy;
-      float tf = (float)tau;
-      if (tf > 1.0) {
-        cx = (float)(cx * 0.1);
-        cy = (float)(cy * 0.1);
-      } else if (tf < 0.1) {
-        cx = (float)(cx * 10.0);
-        cy = (float)(cy * 10.0);
-      } else {
-        float temp = (float)(1.0 / pow(10.0, ((double)tf - 0.1) * 20 / 9.0 - 1.0));
-        cx *= temp;
-        cy *= temp;
-      }
-      sc->inv_gamma_x = cx;
-      sc->inv_gamma_y = cy;
-      *ticket = 0;
+}
+
+// tau = overflow area / node area, mean rho, and the gamma schedule
+// (eplace.cpp:475-500) in the reference's float arithmetic.
+__global__ void __launch_bounds__(kBlock) finish_density_kernel(int nblocks, int nb, Grid g,
+                                                                 double node_area_scaled, const double *partials,
+                                                                 DevScalars *sc) {
+  if (sc->done) return;
+  double tot = block_sum_partials(partials + kPartialTau, nblocks);
+  double rtot = block_sum_partials(partials + kPartialRho, nblocks);
+  if (threadIdx.x == 0) {
+    double tau = tot / node_area_scaled;
+    sc->overflow = tau;
+    sc->rho_mean = rtot / (double)nb;
+    float cx = 0.125f / g.bsx, cy = 0.125f / g.bsy;
+    float tf = (float)tau;
+    if (tf > 1.0) {
+      cx = (float)(cx * 0.1);
+      cy = (float)(cy * 0.1);
+    } else if (tf < 0.1) {
+      cx = (float)(cx * 10.0);
+      cy = (float)(cy * 10.0);
+    } else {
+      float temp = (float)(1.0 / pow(10.0, ((double)tf - 0.1) * 20 / 9.0 - 1.0));
+      cx *= temp;
+      cy *= temp;
     }
+    sc->inv_gamma_x = cx;
+    sc->inv_gamma_y = cy;
   }
 }
 
@@ -296,11 +298,11 @@ __global__ void __launch_bounds__(kBlock) gather_combine_kernel(
   }
 }
 
-// lambda0 = sum |gwl| / sum |gd| (eplace.cpp:812-835), partial sums.
+// lambda0 = sum |gwl| / sum |gd| (eplace.cpp:812-835): partial sums here,
+// the quotient in lambda_init_finish_kernel.
 __global__ void __launch_bounds__(kBlock) lambda_init_kernel(int O, int N, const float2 *__restrict__ gd,
                                                               const float2 *__restrict__ gwl,
-                                                              double *partials, unsigned int *ticket,
-                                                              DevScalars *sc) {
+                                                              double *partials) {
   int o = blockIdx.x * blockDim.x + threadIdx.x;
   double a = 0, b = 0;
   if (o < O) {
@@ -317,14 +319,15 @@ __global__ void __launch_bounds__(kBlock) lambda_init_kernel(int O, int N, const
     partials[kPartialLambda + 2 * blockIdx.x] = a;
     partials[kPartialLambda + 2 * blockIdx.x + 1] = b;
   }
-  if (last_block_done(ticket)) {
-    double na = block_sum_partials_strided(partials + kPartialLambda, gridDim.x, 2);
-    double nb = block_sum_partials_strided(partials + kPartialLambda + 1, gridDim.x, 2);
-    if (threadIdx.x == 0) {
-      sc->lambda = na / nb;
-      sc->last_hpwl = sc->hpwl;
-      *ticket = 0;
-    }
+}
+
+__global__ void __launch_bounds__(kBlock) lambda_init_finish_kernel(int nblocks, const double *partials,
+                                                                     DevScalars *sc) {
+  double na = block_sum_partials_strided(partials + kPartialLambda, nblocks, 2);
+  double nb = block_sum_partials_strided(partials + kPartialLambda + 1, nblocks, 2);
+  if (threadIdx.x == 0) {
+    sc->lambda = na / nb;
+    sc->last_hpwl = sc->hpwl;
   }
 }
 
@@ -354,8 +357,10 @@ void launch_finalize_density(Ctx &c) {
   int nb = c.nx * c.ny;
   double nas = (double)(c.std_area + c.macro_area * c.td);
   hipLaunchKernelGGL(finalize_density_kernel, dim3(blocks_for(nb)), dim3(kBlock), 0, c.stream, nb, c.node_q,
-                     c.filler_q, c.term_map, c.base_map, g, poisson_ld(c), ldexp(1.0, -c.shift), nas, c.rho,
-                     c.partials, c.tickets + 0, c.sc);
+                     c.filler_q, c.term_map, c.base_map, g, poisson_ld(c), ldexp(1.0, -c.shift), c.rho,
+                     c.partials, c.sc);
+  hipLaunchKernelGGL(finish_density_kernel, dim3(1), dim3(kBlock), 0, c.stream, blocks_for(nb), nb, g, nas,
+                     c.partials, c.sc);
 }
 
 void launch_gather_combine(Ctx &c, bool recompute, bool lipschitz, const float2 *ref_now,
@@ -373,7 +378,9 @@ void launch_gather_combine(Ctx &c, bool recompute, bool lipschitz, const float2 
 
 void launch_lambda_init(Ctx &c) {
   hipLaunchKernelGGL(lambda_init_kernel, dim3(blocks_for(c.O)), dim3(kBlock), 0, c.stream, c.O, c.N, c.gd,
-                     c.gwl, c.partials, c.tickets + 1, c.sc);
+                     c.gwl, c.partials);
+  hipLaunchKernelGGL(lambda_init_finish_kernel, dim3(1), dim3(kBlock), 0, c.stream, blocks_for(c.O), c.partials,
+                     c.sc);
 }
 
 }  // namespace ep

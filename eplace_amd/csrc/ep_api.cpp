@@ -357,15 +357,28 @@ int create(Ctx &c, const ep_design *d, const ep_params *p) {
     }
   }
   c.num_big = (int)big.size();
-  // ---- nets split by degree ----
-  std::vector<int32_t> small, large, ns32(c.E + 1);
-  for (int e = 0; e < c.E; ++e) {
-    int64_t deg = d->net_start[e + 1] - d->net_start[e];
-    if (deg == 0) continue;
-    (deg <= kSmallNetMaxDegree ? small : large).push_back(e);
+  // ---- nets: degree > 64 get a wave each; runs of consecutive smaller nets
+  // are packed into groups of at most 64 pins (one wave, one pin per lane) ----
+  std::vector<int32_t> large, gbeg, gend, ns32(c.E + 1), pin_net(std::max<int64_t>(c.P, 1));
+  {
+    int32_t cur_b = -1, cur_e = -1;
+    for (int e = 0; e < c.E; ++e) {
+      int32_t s0 = (int32_t)d->net_start[e], s1 = (int32_t)d->net_start[e + 1];
+      for (int32_t q = s0; q < s1; ++q) pin_net[q] = e;
+      if (s1 == s0) continue;
+      if (s1 - s0 > kSmallNetMaxDegree) {
+        large.push_back(e);
+        if (cur_b >= 0) { gbeg.push_back(cur_b); gend.push_back(cur_e); cur_b = -1; }
+        continue;
+      }
+      if (cur_b >= 0 && (s1 - cur_b > 64 || s0 != cur_e)) { gbeg.push_back(cur_b); gend.push_back(cur_e); cur_b = -1; }
+      if (cur_b < 0) cur_b = s0;
+      cur_e = s1;
+    }
+    if (cur_b >= 0) { gbeg.push_back(cur_b); gend.push_back(cur_e); }
   }
   for (int e = 0; e <= c.E; ++e) ns32[e] = (int32_t)d->net_start[e];
-  c.num_small = (int)small.size();
+  c.num_groups = (int)gbeg.size();
   c.num_large = (int)large.size();
   // ---- node -> pins CSR in modulePins (file) order ----
   std::vector<int32_t> nps(c.N + 1, 0), npl(std::max<int64_t>(c.P, 1));
@@ -388,7 +401,9 @@ int create(Ctx &c, const ep_design *d, const ep_params *p) {
             dev_upload(c, c.tx, tx.data(), c.NT) && dev_upload(c, c.ty, ty.data(), c.NT) &&
             dev_upload(c, c.net_start, ns32.data(), c.E + 1) && dev_upload(c, c.pin_obj, d->pin_obj, c.P) &&
             dev_upload(c, c.pin_dx, d->pin_dx, c.P) && dev_upload(c, c.pin_dy, d->pin_dy, c.P) &&
-            dev_upload(c, c.small_nets, small.data(), small.size()) &&
+            dev_upload(c, c.group_begin, gbeg.data(), gbeg.size()) &&
+            dev_upload(c, c.group_end, gend.data(), gend.size()) &&
+            dev_upload(c, c.pin_net, pin_net.data(), pin_net.size()) &&
             dev_upload(c, c.large_nets, large.data(), large.size()) &&
             dev_upload(c, c.node_pin_start, nps.data(), c.N + 1) &&
             dev_upload(c, c.node_pins, npl.data(), npl.size()) && dev_zero(c, c.pin_grad, c.P) &&
@@ -421,7 +436,7 @@ void destroy(Ctx &c) {
   if (c.stream) (void)hipStreamSynchronize(c.stream);
   poisson_destroy(c);
   void *ptrs[] = {c.x, c.y, c.w, c.h, c.area, c.kind, c.in_stage, c.npins, c.big, c.is_big, c.tx, c.ty,
-                  c.net_start, c.pin_obj, c.pin_dx, c.pin_dy, c.small_nets, c.large_nets, c.node_pin_start,
+                  c.net_start, c.pin_obj, c.pin_dx, c.pin_dy, c.group_begin, c.group_end, c.pin_net, c.large_nets, c.node_pin_start,
                   c.node_pins, c.pin_grad, c.bllx, c.burx, c.blly, c.bury, c.node_q, c.filler_q, c.term_map,
                   c.base_map, c.rho, c.phi, c.ex, c.ey, c.gd, c.gwl, c.gtot, c.cur_main, c.cur_ref, c.last_ref,
                   c.last_grad, c.new_main, c.new_ref, c.cur_grad, c.sc, c.partials, c.tickets};
@@ -429,6 +444,7 @@ void destroy(Ctx &c) {
     if (p) (void)hipFree(p);
   for (auto &e : c.ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto &e : c.prof_events) (void)hipEventDestroy(e);
   if (c.stream) (void)hipStreamDestroy(c.stream);
   delete c.rng;
 }
@@ -436,20 +452,27 @@ void destroy(Ctx &c) {
 // One full evaluation of totalGradientUpdate at the current positions.
 void evaluate(Ctx &c, bool lambda_update, bool lipschitz, const float2 *ref_now, const float2 *ref_prev,
               const float2 *grad_prev, bool timed) {
-  if (timed) (void)hipEventRecord(c.ev[0], c.stream);
+  hipEvent_t *ev = nullptr;
+  if (timed) {
+    ev = c.ev;
+  } else if (c.profiling && c.prof_used + 6 <= c.prof_events.size()) {
+    ev = &c.prof_events[c.prof_used];
+    c.prof_used += 6;
+  }
+  if (ev) (void)hipEventRecord(ev[0], c.stream);
   launch_zero_maps(c);
   launch_scatter(c);
-  if (timed) (void)hipEventRecord(c.ev[1], c.stream);
+  if (ev) (void)hipEventRecord(ev[1], c.stream);
   launch_finalize_density(c);
-  if (timed) (void)hipEventRecord(c.ev[2], c.stream);
+  if (ev) (void)hipEventRecord(ev[2], c.stream);
   launch_wirelength(c, true);
   launch_reduce_wirelength(c, lambda_update ? 1 : 0);
-  if (timed) (void)hipEventRecord(c.ev[3], c.stream);
+  if (ev) (void)hipEventRecord(ev[3], c.stream);
   launch_poisson(c, false);
-  if (timed) (void)hipEventRecord(c.ev[4], c.stream);
+  if (ev) (void)hipEventRecord(ev[4], c.stream);
   launch_gather_combine(c, true, lipschitz, ref_now, ref_prev, grad_prev);
   launch_count_eval(c);
-  if (timed) (void)hipEventRecord(c.ev[5], c.stream);
+  if (ev) (void)hipEventRecord(ev[5], c.stream);
 }
 
 int sync(Ctx &c) {
@@ -544,6 +567,18 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
     EP_HIP(hipMalloc(&dlog, (size_t)cap * 8 * sizeof(double)));
     EP_HIP(hipMemset(dlog, 0, (size_t)cap * 8 * sizeof(double)));
   }
+  if (o.phase_seconds) {
+    // events for every evaluation, created before the timed loop
+    size_t need = (size_t)6 * ((size_t)o.max_iterations + 2) * (o.method == 1 ? 3 : 1);
+    need = std::min<size_t>(need, 6 * 20000);
+    while (c.prof_events.size() < need) {
+      hipEvent_t e;
+      EP_HIP(hipEventCreate(&e));
+      c.prof_events.push_back(e);
+    }
+    c.prof_used = 0;
+    c.profiling = true;
+  }
   auto t0 = std::chrono::steady_clock::now();
   // init (nesterov.hpp:207-214): a_0 = 1, main = current positions
   int rc = update_scalars(c, [&](DevScalars &h) {
@@ -633,6 +668,19 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
     if (out) out->final_hpwl = h3.hpwl;
   }
   auto t1 = std::chrono::steady_clock::now();
+  if (c.profiling) {
+    c.profiling = false;
+    double acc[5] = {0, 0, 0, 0, 0};
+    size_t n = c.prof_used / 6;
+    for (size_t e = 0; e < n; ++e)
+      for (int k = 0; k < 5; ++k) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, c.prof_events[6 * e + k], c.prof_events[6 * e + k + 1]);
+        acc[k] += ms * 1e-3;
+      }
+    for (int k = 0; k < 5; ++k) o.phase_seconds[k] = acc[k];
+    o.phase_seconds[5] = (double)n;
+  }
   if (cap > 0) {
     EP_HIP(hipMemcpy(o.log, dlog, (size_t)cap * 8 * sizeof(double), hipMemcpyDeviceToHost));
     (void)hipFree(dlog);

@@ -15,7 +15,7 @@ enum ObjKind : uint8_t { KIND_CELL = 0, KIND_MACRO = 1, KIND_FILLER = 2 };
 
 constexpr double kEps = 1.0e-15;  // global.h:39
 constexpr int kBlock = 256;
-constexpr int kSmallNetMaxDegree = 32;  // nets above this go to the wave-per-net kernel
+constexpr int kSmallNetMaxDegree = 64;  // nets above this get a wave each; the rest are packed 64 pins per wave
 constexpr int kBigObjectBins = 16;      // objects that may cover more bins get a block each
 
 // Bin grid, passed by value to the density kernels.  Edges are the
@@ -61,6 +61,10 @@ struct Ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev[6] = {};
   double last_timings[5] = {};
+  // event pool for per-phase timing inside the optimiser loop (6 per evaluation)
+  std::vector<hipEvent_t> prof_events;
+  size_t prof_used = 0;
+  bool profiling = false;
 
   // sizes
   int N = 0, F = 0, O = 0, NT = 0, E = 0;
@@ -96,8 +100,11 @@ struct Ctx {
   int32_t *net_start = nullptr;  // [E+1]
   int32_t *pin_obj = nullptr;
   float *pin_dx = nullptr, *pin_dy = nullptr;
-  int32_t *small_nets = nullptr, *large_nets = nullptr;
-  int num_small = 0, num_large = 0;
+  int32_t *large_nets = nullptr;
+  int num_large = 0;
+  int32_t *group_begin = nullptr, *group_end = nullptr;  // pin ranges of the 64-pin net groups
+  int num_groups = 0;
+  int32_t *pin_net = nullptr;  // net of each pin
   int32_t *node_pin_start = nullptr;  // [N+1]
   int32_t *node_pins = nullptr;
   float2 *pin_grad = nullptr;

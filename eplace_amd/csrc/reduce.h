@@ -1,4 +1,6 @@
 // Deterministic block reductions (fixed shuffle tree, fixed partial order).
+// Per-block partials are written by one kernel and summed by the next, so
+// kernel boundaries order them; no in-kernel fences are needed.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -31,30 +33,15 @@ __device__ __forceinline__ double block_sum(double v) {
   return r;
 }
 
-// True in every thread of the block that finishes last.  Call after the
-// block's partial result has been stored.
-__device__ __forceinline__ bool last_block_done(unsigned int *ticket) {
-  __shared__ int is_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned int t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    is_last = (t == gridDim.x - 1) ? 1 : 0;
-  }
-  __syncthreads();
-  if (is_last) __threadfence();
-  return is_last != 0;
-}
-
-__device__ __forceinline__ double block_sum_partials(const double *p, int n) {
+__device__ __forceinline__ double block_sum_partials(const double *__restrict__ p, int n) {
   double s = 0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) s += __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
   return block_sum(s);
 }
 
 __device__ __forceinline__ double block_sum_partials_strided(const double *p, int n, int stride) {
   double s = 0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) s += __hip_atomic_load(p + (size_t)i * stride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[(size_t)i * stride];
   return block_sum(s);
 }
 

@@ -54,15 +54,18 @@ __device__ __forceinline__ void pin_terms(float x, float xmax, float xmin, float
   em = wl_exp<EXPM>((xmin - x) * igx);
 }
 
-// Pin gradient along one axis from the net sums.
+// Pin gradient along one axis from the net sums, evaluated in FP64: the
+// factor 1 - ig (v - N'-/D-) cancels when ig * (pin spread) ~ 1, which in
+// float alone costs ~1e-5 of the node gradient on small 2-pin designs.
 template <int MODEL>
 __device__ __forceinline__ float pin_grad_axis(float x, float xmax, float xmin, float ig, float ep, float em,
                                                float dp, float np, float dm, float nm) {
-  if (MODEL == 1) return ep / dp - em / dm;  // LSE
-  float gp = 0.f, gm = 0.f;
-  if (ep != 0.f) gp = ep / dp * (1.f + ig * ((x - xmax) - np / dp));
-  if (em != 0.f) gm = em / dm * (1.f - ig * ((x - xmin) - nm / dm));
-  return gp - gm;
+  if (MODEL == 1) return (float)((double)ep / dp - (double)em / dm);  // LSE
+  double gp = 0.0, gm = 0.0;
+  const double g = ig;
+  if (ep != 0.f) gp = (double)ep / dp * (1.0 + g * ((double)(x - xmax) - (double)np / dp));
+  if (em != 0.f) gm = (double)em / dm * (1.0 - g * ((double)(x - xmin) - (double)nm / dm));
+  return (float)(gp - gm);
 }
 
 template <int MODEL>
@@ -71,64 +74,6 @@ __device__ __forceinline__ double net_value(float xmax, float xmin, float ig, fl
   if (MODEL == 1)
     return (double)(xmax - xmin) + (double)(logf(dp) / ig) + (double)(logf(dm) / ig);
   return ((double)xmax + (double)(np / dp)) - ((double)xmin + (double)(nm / dm));
-}
-
-// One thread per net (degree <= kSmallNetMaxDegree), grid-stride.
-template <int MODEL, int EXPM, int GRAD>
-__global__ void __launch_bounds__(kBlock) wl_small_kernel(int num, const int32_t *__restrict__ nets,
-                                                           PinSrc src, const DevScalars *sc,
-                                                           float2 *__restrict__ pin_grad, double *partials,
-                                                           int slot0) {
-  if (sc->done) return;
-  float igx = (float)sc->inv_gamma_x, igy = (float)sc->inv_gamma_y;
-  double hp = 0, wv = 0;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < num; t += gridDim.x * blockDim.x) {
-    int e = nets ? nets[t] : t;
-    int s = src.net_start[e], n = src.net_start[e + 1];
-    if (s == n) continue;
-    float xmax = -INFINITY, xmin = INFINITY, ymax = -INFINITY, ymin = INFINITY;
-    for (int p = s; p < n; ++p) {
-      float px = src.px(p), py = src.py(p);
-      xmax = fmaxf(xmax, px);
-      xmin = fminf(xmin, px);
-      ymax = fmaxf(ymax, py);
-      ymin = fminf(ymin, py);
-    }
-    hp += ((double)xmax - (double)xmin) + ((double)ymax - (double)ymin);
-    if (!GRAD) continue;
-    NetAcc a = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int p = s; p < n; ++p) {
-      float px = src.px(p), py = src.py(p);
-      float ep, em, fp, fm;
-      pin_terms<MODEL, EXPM>(px, xmax, xmin, igx, ep, em);
-      pin_terms<MODEL, EXPM>(py, ymax, ymin, igy, fp, fm);
-      a.dpx += ep;
-      a.npx += (px - xmax) * ep;
-      a.dmx += em;
-      a.nmx += (px - xmin) * em;
-      a.dpy += fp;
-      a.npy += (py - ymax) * fp;
-      a.dmy += fm;
-      a.nmy += (py - ymin) * fm;
-    }
-    wv += net_value<MODEL>(xmax, xmin, igx, a.dpx, a.npx, a.dmx, a.nmx) +
-          net_value<MODEL>(ymax, ymin, igy, a.dpy, a.npy, a.dmy, a.nmy);
-    for (int p = s; p < n; ++p) {
-      float px = src.px(p), py = src.py(p);
-      float ep, em, fp, fm;
-      pin_terms<MODEL, EXPM>(px, xmax, xmin, igx, ep, em);
-      pin_terms<MODEL, EXPM>(py, ymax, ymin, igy, fp, fm);
-      float gx = pin_grad_axis<MODEL>(px, xmax, xmin, igx, ep, em, a.dpx, a.npx, a.dmx, a.nmx);
-      float gy = pin_grad_axis<MODEL>(py, ymax, ymin, igy, fp, fm, a.dpy, a.npy, a.dmy, a.nmy);
-      pin_grad[p] = make_float2(gx, gy);
-    }
-  }
-  hp = block_sum(hp);
-  wv = block_sum(wv);
-  if (threadIdx.x == 0) {
-    partials[kPartialWL + 2 * (slot0 + blockIdx.x)] = hp;
-    partials[kPartialWL + 2 * (slot0 + blockIdx.x) + 1] = wv;
-  }
 }
 
 __device__ __forceinline__ float wave_max(float v) {
@@ -144,7 +89,7 @@ __device__ __forceinline__ float wave_fsum(float v) {
   return v;
 }
 
-// One wave per large net, grid-stride over nets.
+// One wave per net of degree > 64, grid-stride over nets.
 template <int MODEL, int EXPM, int GRAD>
 __global__ void __launch_bounds__(kBlock) wl_large_kernel(int num, const int32_t *__restrict__ nets,
                                                            PinSrc src, const DevScalars *sc,
@@ -216,6 +161,90 @@ __global__ void __launch_bounds__(kBlock) wl_large_kernel(int num, const int32_t
   }
 }
 
+// Segmented reductions inside a wave over runs of lanes holding the same
+// net.  same[k] says lane+2^k holds the same net as this lane.
+template <class Op>
+__device__ __forceinline__ float seg_reduce(float v, const bool *same, Op op) {
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    float o = __shfl_down(v, 1 << k, 64);
+    if (same[k]) v = op(v, o);
+  }
+  return v;
+}
+
+// One wave per group of consecutive nets of total degree <= 64, one pin per
+// lane (coalesced pin loads, no divergence on net degree).  Bound pins,
+// HPWL, the WA/LSE sums and the per-pin gradient come from segmented
+// shuffle reductions over each net's lanes; the net's head lane holds the
+// result and broadcasts it.
+template <int MODEL, int EXPM, int GRAD>
+__global__ void __launch_bounds__(kBlock) wl_group_kernel(int num_groups, const int32_t *__restrict__ gbeg,
+                                                           const int32_t *__restrict__ gend,
+                                                           const int32_t *__restrict__ pin_net, PinSrc src,
+                                                           const DevScalars *sc, float2 *__restrict__ pin_grad,
+                                                           double *partials) {
+  if (sc->done) return;
+  const float igx = (float)sc->inv_gamma_x, igy = (float)sc->inv_gamma_y;
+  const int lane = threadIdx.x & 63;
+  const int wpb = blockDim.x >> 6;
+  double hp = 0, wv = 0;
+  for (int gi = blockIdx.x * wpb + (threadIdx.x >> 6); gi < num_groups; gi += gridDim.x * wpb) {
+    const int p0 = gbeg[gi], p1 = gend[gi];
+    const int p = p0 + lane;
+    const bool act = p < p1;
+    int net = act ? pin_net[p] : -1 - lane;  // inactive lanes: unique segments
+    float px = 0.f, py = 0.f;
+    if (act) {
+      px = src.px(p);
+      py = src.py(p);
+    }
+    bool same[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      int on = __shfl_down(net, 1 << k, 64);
+      same[k] = (lane + (1 << k) < 64) && on == net;
+    }
+    int head = act ? src.net_start[net] - p0 : lane;
+    auto fmax_ = [](float a, float b) { return fmaxf(a, b); };
+    auto fmin_ = [](float a, float b) { return fminf(a, b); };
+    auto fadd_ = [](float a, float b) { return a + b; };
+    float xmax = __shfl(seg_reduce(px, same, fmax_), head, 64);
+    float xmin = __shfl(seg_reduce(px, same, fmin_), head, 64);
+    float ymax = __shfl(seg_reduce(py, same, fmax_), head, 64);
+    float ymin = __shfl(seg_reduce(py, same, fmin_), head, 64);
+    const bool is_head = act && head == lane;
+    if (is_head) hp += ((double)xmax - (double)xmin) + ((double)ymax - (double)ymin);
+    if (!GRAD) continue;
+    float ep = 0.f, em = 0.f, fp = 0.f, fm = 0.f;
+    if (act) {
+      pin_terms<MODEL, EXPM>(px, xmax, xmin, igx, ep, em);
+      pin_terms<MODEL, EXPM>(py, ymax, ymin, igy, fp, fm);
+    }
+    float dpx = __shfl(seg_reduce(ep, same, fadd_), head, 64);
+    float npx = __shfl(seg_reduce((px - xmax) * ep, same, fadd_), head, 64);
+    float dmx = __shfl(seg_reduce(em, same, fadd_), head, 64);
+    float nmx = __shfl(seg_reduce((px - xmin) * em, same, fadd_), head, 64);
+    float dpy = __shfl(seg_reduce(fp, same, fadd_), head, 64);
+    float npy = __shfl(seg_reduce((py - ymax) * fp, same, fadd_), head, 64);
+    float dmy = __shfl(seg_reduce(fm, same, fadd_), head, 64);
+    float nmy = __shfl(seg_reduce((py - ymin) * fm, same, fadd_), head, 64);
+    if (is_head)
+      wv += net_value<MODEL>(xmax, xmin, igx, dpx, npx, dmx, nmx) + net_value<MODEL>(ymax, ymin, igy, dpy, npy, dmy, nmy);
+    if (act) {
+      float gx = pin_grad_axis<MODEL>(px, xmax, xmin, igx, ep, em, dpx, npx, dmx, nmx);
+      float gy = pin_grad_axis<MODEL>(py, ymax, ymin, igy, fp, fm, dpy, npy, dmy, nmy);
+      pin_grad[p] = make_float2(gx, gy);
+    }
+  }
+  hp = block_sum(hp);
+  wv = block_sum(wv);
+  if (threadIdx.x == 0) {
+    partials[kPartialWL + 2 * blockIdx.x] = hp;
+    partials[kPartialWL + 2 * blockIdx.x + 1] = wv;
+  }
+}
+
 // Sums the per-block HPWL/WL partials in a fixed order; optionally applies
 // updatePenaltyFactor (eplace.cpp:846-883) with FP64 HPWL and lambda.
 __global__ void __launch_bounds__(kBlock) reduce_wl_kernel(int nslots, double *partials, DevScalars *sc,
@@ -237,7 +266,8 @@ __global__ void __launch_bounds__(kBlock) reduce_wl_kernel(int nslots, double *p
 }
 
 static int g_small_blocks(const Ctx &c) {
-  int b = (c.num_small + kBlock - 1) / kBlock;
+  int wpb = kBlock / 64;
+  int b = (c.num_groups + wpb - 1) / wpb;
   return b < 1 ? 1 : (b > 8192 ? 8192 : b);
 }
 static int g_large_blocks(const Ctx &c) {
@@ -250,8 +280,8 @@ template <int MODEL, int EXPM, int GRAD>
 static void launch_wl_t(Ctx &c) {
   PinSrc src{c.net_start, c.pin_obj, c.pin_dx, c.pin_dy, c.x, c.y, c.tx, c.ty, c.N};
   int bs = g_small_blocks(c);
-  hipLaunchKernelGGL((wl_small_kernel<MODEL, EXPM, GRAD>), dim3(bs), dim3(kBlock), 0, c.stream, c.num_small,
-                     c.small_nets, src, c.sc, c.pin_grad, c.partials, 0);
+  hipLaunchKernelGGL((wl_group_kernel<MODEL, EXPM, GRAD>), dim3(bs), dim3(kBlock), 0, c.stream, c.num_groups,
+                     c.group_begin, c.group_end, c.pin_net, src, c.sc, c.pin_grad, c.partials);
   if (c.num_large) {
     hipLaunchKernelGGL((wl_large_kernel<MODEL, EXPM, GRAD>), dim3(g_large_blocks(c)), dim3(kBlock), 0,
                        c.stream, c.num_large, c.large_nets, src, c.sc, c.pin_grad, c.partials, bs);

@@ -216,15 +216,22 @@ class EplaceNesterovOpt:
         self.log = None
         self.result = None
 
-    def opt(self) -> dict:
+    def opt(self, profile: bool = False) -> dict:
         cap = self.max_iterations + 2
         log = np.zeros((cap, 8), np.float64)
+        phases = np.zeros(6, np.float64)
         p = L.EPOptParams(method=1 if self.method == "bktrk" else 0, target_overflow=self.target_overflow,
                           max_iterations=self.max_iterations, max_backtracks=self.max_backtracks,
-                          sync_every=self.sync_every, log_capacity=cap, log=_ptr(log))
+                          sync_every=self.sync_every, log_capacity=cap, log=_ptr(log),
+                          phase_seconds=_ptr(phases) if profile else None)
         r = L.EPRunStats()
         self.placer._check(self.placer.lib.ep_nesterov_run(self.placer.h, C.byref(p), C.byref(r)))
         res = {k: getattr(r, k) for k, _ in L.EPRunStats._fields_}
+        if profile:
+            n = max(phases[5], 1.0)
+            res["phase_seconds"] = dict(zip(["scatter", "finalize", "wirelength", "poisson", "gather_combine"],
+                                            (phases[:5] / n).tolist()))
+            res["phase_evaluations"] = int(phases[5])
         self.log = log[: res["iterations"]]
         if self.placer.placementStage == MGP:
             self.placer.mGPIterationCount = res["iterations"]

@@ -94,6 +94,9 @@ typedef struct ep_opt_params {
   int32_t sync_every;       /* host polls the stop flag every this many iterations */
   int32_t log_capacity;     /* rows available in log (may be 0) */
   double *log;              /* optional [log_capacity][8]: iter, tau, lambda, hpwl, step, invgx, evals, backtracks */
+  double *phase_seconds;    /* optional [6]: summed kernel time of the full evaluations per phase
+                               (scatter, density finalise, wirelength, poisson, gather+combine) and
+                               the number of evaluations timed; HIP events on the engine's stream */
 } ep_opt_params;
 
 typedef struct ep_run_stats {

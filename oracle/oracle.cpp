@@ -97,6 +97,21 @@ inline T fast_exp(T a) {
 // (Makhoul's even/odd reordering).  Used for the Poisson solve at sizes where
 // the O(N^3) matrix form is too slow.
 // ---------------------------------------------------------------------------
+// Twiddles e^{-2 pi i k / n}, k < n/2, computed in double, cached per n.
+template <class T>
+const std::vector<std::complex<T>> &twiddles(size_t n) {
+  static thread_local std::vector<std::complex<T>> tw[32];
+  int lg = 0;
+  while (((size_t)1 << lg) < n) ++lg;
+  std::vector<std::complex<T>> &t = tw[lg];
+  if (t.size() != n / 2) {
+    t.resize(n / 2);
+    for (size_t k = 0; k < n / 2; ++k)
+      t[k] = std::complex<T>((T)std::cos(2 * M_PI * k / n), (T)-std::sin(2 * M_PI * k / n));
+  }
+  return t;
+}
+
 template <class T>
 void fft_inplace(std::vector<std::complex<T>> &a, bool inverse) {
   size_t n = a.size();
@@ -106,11 +121,13 @@ void fft_inplace(std::vector<std::complex<T>> &a, bool inverse) {
     j ^= bit;
     if (i < j) std::swap(a[i], a[j]);
   }
+  const std::vector<std::complex<T>> &tw = twiddles<T>(n);
   for (size_t len = 2; len <= n; len <<= 1) {
-    double ang = 2 * M_PI / (double)len * (inverse ? 1 : -1);
+    size_t step = n / len;
     for (size_t i = 0; i < n; i += len)
       for (size_t j = 0; j < len / 2; ++j) {
-        std::complex<T> w((T)std::cos(ang * j), (T)std::sin(ang * j));
+        std::complex<T> w = tw[j * step];
+        if (inverse) w = std::conj(w);
         std::complex<T> u = a[i + j], v = a[i + j + len / 2] * w;
         a[i + j] = u + v;
         a[i + j + len / 2] = u - v;

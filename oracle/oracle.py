@@ -128,6 +128,10 @@ class Oracle:
     def set(self, name, values):
         dt = np.int64 if name.endswith("_q") else np.float64
         v = np.ascontiguousarray(values, dtype=dt).reshape(-1)
+        want = {"stage_positions_clamped": "stage_positions"}.get(name, name)
+        n = lib().orc_get(self.h, want.encode(), None)
+        if n >= 0 and v.size != n:
+            raise ValueError(f"{name}: got {v.size} values, the oracle holds {n}")
         if not lib().orc_set(self.h, name.encode(), v.ctypes.data):
             raise KeyError(name)
 

@@ -52,15 +52,31 @@ def test_first_evaluation_vs_reference(name):
     assert errs["node"] < 1e-6 and errs["filler"] < 1e-6
     assert errs["ex"] < 1e-5 and errs["ey"] < 1e-5
     assert errs["grad_density"] < 1e-5
-    # the reference's float WA quotient cancels when |x| >> gamma; the GPU's
-    # shifted form does not, so the bar here is the reference's own error
-    assert errs["grad_wirelength"] < 1e-3
+    # The reference's float WA quotient ((e + x ig e) D - ig e N) / D^2
+    # cancels when |x| >> 1/ig; the GPU's shifted form does not.  Bound the
+    # GPU-reference gap by the reference's own distance from exact FP64 (same
+    # positions and gamma), and the GPU's distance from FP64 by 1e-5.
+    from oracle.oracle import Oracle
+    o = Oracle(g.design(), g.target_density, mode="fp64")
+    o.set("positions", gp.read("positions").reshape(-1).astype(np.float64))
+    o.set_scalar("invgx", st["inv_gamma_x"])
+    o.set_scalar("invgy", st["inv_gamma_y"])
+    o.run("wirelength")
+    exact = o.get("grad_wirelength").reshape(-1, 2)
+    ref_err = rel(g["init_grad_wirelength"], exact)
+    gpu_err = rel(gp.read("grad_wirelength"), exact)
+    say(f"  wirelength gradient vs FP64: reference {ref_err:.2e}, gpu {gpu_err:.2e}")
+    assert gpu_err < 1e-5
+    assert errs["grad_wirelength"] < ref_err * 1.05 + 1e-5
+    # tau is an FP64 sum on the GPU (float in the reference); 1/gamma is a
+    # continuous function of it
     assert abs(st["overflow"] - g.s["init_overflow"]) < 1e-6 * g.s["init_overflow"]
-    assert np.float32(st["inv_gamma_x"]) == np.float32(g.s["init_inv_gamma_x"])
+    assert abs(st["inv_gamma_x"] - g.s["init_inv_gamma_x"]) < 1e-5 * g.s["init_inv_gamma_x"]
     gp.penaltyFactorInitilization()
     st = gp.stats()
     assert abs(st["last_hpwl"] - g.s["hpwl0"]) < 1e-9 * g.s["hpwl0"]
-    assert abs(st["lambda"] - g.s["lambda0"]) < 1e-4 * g.s["lambda0"]
+    # lambda0 = sum|grad WL| / sum|grad D| inherits the reference's WA error
+    assert abs(st["lambda"] - g.s["lambda0"]) < (ref_err * 1.05 + 1e-5) * g.s["lambda0"]
 
 
 # final-HPWL bars as in tests/test_oracle_golden.py (1.5% on the 100-cell
