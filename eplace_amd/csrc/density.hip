@@ -3,6 +3,10 @@
 // Compiled with -ffp-contract=off: the footprint and the per-(object, bin)
 // contribution reproduce the reference's float/double arithmetic exactly, so
 // the int64 fixed-point maps are bit-identical to the CPU oracle's.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+
 #include "ep_internal.h"
 #include "reduce.h"
 
@@ -96,6 +100,119 @@ __global__ void __launch_bounds__(kBlock) scatter_kernel(int O, const float *__r
       if (ov == 0) continue;
       atomicAdd(dst + (size_t)i * g.ny + j, fixed_contribution(s, ov, scale));
     }
+}
+
+// ---------------------------------------------------------------------------
+// Tile-sorted scatter.  Hot bins (clustered placements put hundreds of
+// contributions on one bin) serialise HBM/L2 atomics, so objects are sorted
+// by (kind, 32x32-bin tile of their footprint's lower-left bin) every
+// evaluation, each chunk of <= kTileChunk objects of one tile accumulates in
+// an LDS window of the tile plus a kTileHalo-bin high-side halo, and only the
+// window's nonzero bins are added to HBM.  Footprint bins outside the window
+// (objects wider than the halo) go straight to HBM.  Integer sums make the
+// result independent of the order.
+// ---------------------------------------------------------------------------
+constexpr int kTile = 32, kTileHalo = 3, kTileWin = kTile + kTileHalo, kTileChunk = 1024;
+
+__global__ void __launch_bounds__(kBlock) tile_key_kernel(int O, const float *__restrict__ x,
+                                                           const float *__restrict__ y,
+                                                           const float *__restrict__ w,
+                                                           const float *__restrict__ h,
+                                                           const uint8_t *__restrict__ kind,
+                                                           const uint8_t *__restrict__ is_big, Grid g, int tyc,
+                                                           uint32_t *keys, float4 *vals, int *ranges,
+                                                           int nranges) {
+  int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o < nranges) ranges[o] = 0;
+  if (o >= O) return;
+  float cx = x[o], cy = y[o], ww = w[o], hh = h[o];
+  uint32_t key = kTileKeySentinel;
+  if (!is_big[o]) {
+    Foot f = footprint(cx, cy, ww, hh, g);
+    int ti = min(f.i0, g.nx - 1) / kTile, tj = min(f.j0, g.ny - 1) / kTile;
+    key = (uint32_t)kind[o] * kTileKeyStride + (uint32_t)(ti * tyc + tj);
+  }
+  keys[o] = key;
+  vals[o] = make_float4(cx, cy, ww, hh);
+}
+
+// begin/end of each key's run in the sorted order (ranges[2k], ranges[2k+1]).
+__global__ void tile_range_kernel(int O, const uint32_t *__restrict__ keys, int *ranges) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= O) return;
+  uint32_t k = keys[i];
+  if (k >= kTileKeySentinel) return;
+  if (i == 0 || keys[i - 1] != k) ranges[2 * k] = i;
+  if (i == O - 1 || keys[i + 1] != k) ranges[2 * k + 1] = i + 1;
+}
+
+// One block: chunks of <= kTileChunk objects per key run, as (key, begin, end).
+__global__ void __launch_bounds__(1024) tile_chunk_kernel(int nkeys, const int *__restrict__ ranges,
+                                                           int4 *chunks, int *nchunks) {
+  __shared__ int part[1024];
+  const int per = (nkeys + blockDim.x - 1) / blockDim.x;
+  const int k0 = threadIdx.x * per, k1 = min(k0 + per, nkeys);
+  int cnt = 0;
+  for (int k = k0; k < k1; ++k) {
+    int n = ranges[2 * k + 1] - ranges[2 * k];
+    cnt += (n + kTileChunk - 1) / kTileChunk;
+  }
+  part[threadIdx.x] = cnt;
+  __syncthreads();
+  for (int off = 1; off < (int)blockDim.x; off <<= 1) {  // inclusive scan
+    int v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int pos = part[threadIdx.x] - cnt;
+  for (int k = k0; k < k1; ++k) {
+    int b = ranges[2 * k], e = ranges[2 * k + 1];
+    for (int c = b; c < e; c += kTileChunk) chunks[pos++] = make_int4(k, c, min(c + kTileChunk, e), 0);
+  }
+  if (threadIdx.x == blockDim.x - 1) *nchunks = part[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(kBlock) tile_scatter_kernel(const int4 *__restrict__ chunks,
+                                                               const int *__restrict__ nchunks,
+                                                               const float4 *__restrict__ vals, Grid g, int tyc,
+                                                               double scale, unsigned long long *node_q,
+                                                               unsigned long long *filler_q) {
+  __shared__ unsigned long long win[kTileWin * kTileWin];
+  const int total = *nchunks;
+  for (int ci = blockIdx.x; ci < total; ci += gridDim.x) {
+    const int4 ch = chunks[ci];
+    const int kind = ch.x / kTileKeyStride, tile = ch.x % kTileKeyStride;
+    const int ib = (tile / tyc) * kTile, jb = (tile % tyc) * kTile;
+    unsigned long long *dst = kind == KIND_FILLER ? filler_q : node_q;
+    for (int t = threadIdx.x; t < kTileWin * kTileWin; t += blockDim.x) win[t] = 0;
+    __syncthreads();
+    for (int q = ch.y + threadIdx.x; q < ch.z; q += blockDim.x) {
+      float4 v = vals[q];
+      Foot f = footprint(v.x, v.y, v.z, v.w, g);
+      float s = f.sx * f.sy;
+      if (kind == KIND_MACRO) s = s * g.td;
+      for (int i = f.i0; i <= f.i1; ++i)
+        for (int j = f.j0; j <= f.j1; ++j) {
+          double ov = bin_overlap(g, i, j, f);
+          if (ov == 0) continue;
+          unsigned long long c = fixed_contribution(s, ov, scale);
+          int li = i - ib, lj = j - jb;
+          if (li >= 0 && li < kTileWin && lj >= 0 && lj < kTileWin)
+            atomicAdd(&win[li * kTileWin + lj], c);
+          else
+            atomicAdd(dst + (size_t)i * g.ny + j, c);
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < kTileWin * kTileWin; t += blockDim.x) {
+      unsigned long long c = win[t];
+      if (c == 0) continue;
+      int i = ib + t / kTileWin, j = jb + t % kTileWin;
+      atomicAdd(dst + (size_t)i * g.ny + j, c);
+    }
+    __syncthreads();
+  }
 }
 
 // Objects covering many bins (macros): one block each, threads over bins.
@@ -343,14 +460,61 @@ void launch_zero_maps(Ctx &c) {
 void launch_scatter(Ctx &c) {
   Grid g = c.grid();
   double scale = ldexp(1.0, c.shift);
-  hipLaunchKernelGGL(scatter_kernel, dim3(blocks_for(c.O)), dim3(kBlock), 0, c.stream, c.O, c.x, c.y, c.w, c.h,
-                     c.kind, c.is_big, g, scale, c.node_q, c.filler_q);
+  TileSort &t = c.tiles;
+  const int nranges = 2 * kTileKeySentinel;
+  hipLaunchKernelGGL(tile_key_kernel, dim3(blocks_for(std::max(c.O, nranges))), dim3(kBlock), 0, c.stream, c.O,
+                     c.x, c.y, c.w, c.h, c.kind, c.is_big, g, t.tyc, t.keys[0], t.vals[0], t.ranges, nranges);
+  hipcub::DoubleBuffer<uint32_t> kb(t.keys[0], t.keys[1]);
+  hipcub::DoubleBuffer<float4> vb(t.vals[0], t.vals[1]);
+  size_t bytes = t.temp_bytes;
+  (void)hipcub::DeviceRadixSort::SortPairs(t.temp, bytes, kb, vb, c.O, 0, kTileKeyBits, c.stream);
+  hipLaunchKernelGGL(tile_range_kernel, dim3(blocks_for(c.O)), dim3(kBlock), 0, c.stream, c.O, kb.Current(),
+                     t.ranges);
+  hipLaunchKernelGGL(tile_chunk_kernel, dim3(1), dim3(1024), 0, c.stream, kTileKeySentinel, t.ranges, t.chunks,
+                     t.nchunks);
+  hipLaunchKernelGGL(tile_scatter_kernel, dim3(t.grid), dim3(kBlock), 0, c.stream, t.chunks, t.nchunks,
+                     vb.Current(), g, t.tyc, scale, c.node_q, c.filler_q);
   if (c.num_big)
     hipLaunchKernelGGL(scatter_big_kernel, dim3(c.num_big), dim3(kBlock), 0, c.stream, c.big, c.x, c.y, c.w,
                        c.h, c.kind, g, scale, c.node_q, c.filler_q);
 }
 
 int poisson_ld(const Ctx &c);
+
+bool tile_sort_create(Ctx &c, std::string &err) {
+  TileSort &t = c.tiles;
+  t.tyc = (c.ny + kTile - 1) / kTile;
+  int txc = (c.nx + kTile - 1) / kTile;
+  if (txc * t.tyc > kTileKeyStride) {
+    err = "grid too large for the tile scatter";
+    return false;
+  }
+  size_t n = std::max(c.O, 1);
+  bool ok = hipMalloc(&t.keys[0], n * 4) == hipSuccess && hipMalloc(&t.keys[1], n * 4) == hipSuccess &&
+            hipMalloc(&t.vals[0], n * 16) == hipSuccess && hipMalloc(&t.vals[1], n * 16) == hipSuccess &&
+            hipMalloc(&t.ranges, 2 * kTileKeySentinel * sizeof(int)) == hipSuccess &&
+            hipMalloc(&t.nchunks, sizeof(int)) == hipSuccess;
+  size_t max_chunks = n / kTileChunk + 3 * (size_t)kTileKeyStride + 1;
+  ok = ok && hipMalloc(&t.chunks, max_chunks * sizeof(int4)) == hipSuccess;
+  if (ok) {
+    hipcub::DoubleBuffer<uint32_t> kb(t.keys[0], t.keys[1]);
+    hipcub::DoubleBuffer<float4> vb(t.vals[0], t.vals[1]);
+    t.temp_bytes = 0;
+    ok = hipcub::DeviceRadixSort::SortPairs(nullptr, t.temp_bytes, kb, vb, (int)n, 0, kTileKeyBits) == hipSuccess &&
+         hipMalloc(&t.temp, std::max<size_t>(t.temp_bytes, 16)) == hipSuccess;
+  }
+  t.grid = (int)std::min<size_t>(max_chunks, 2048);
+  if (!ok) err = "tile scatter allocation failed";
+  return ok;
+}
+
+void tile_sort_destroy(Ctx &c) {
+  TileSort &t = c.tiles;
+  void *p[] = {t.keys[0], t.keys[1], t.vals[0], t.vals[1], t.ranges, t.nchunks, t.chunks, t.temp};
+  for (void *q : p)
+    if (q) (void)hipFree(q);
+  t = TileSort();
+}
 
 void launch_finalize_density(Ctx &c) {
   Grid g = c.grid();

@@ -419,6 +419,7 @@ int create(Ctx &c, const ep_design *d, const ep_params *p) {
   if (!ok) return EP_ERR_NOMEM;
   std::string err;
   if (!poisson_create(c, err)) return c.err = err, EP_ERR_NOMEM;
+  if (!tile_sort_create(c, err)) return c.err = err, EP_ERR_NOMEM;
   rebuild_stage(c);
   DevScalars h = {};
   h.a_k = 1;
@@ -435,6 +436,7 @@ void destroy(Ctx &c) {
   (void)hipSetDevice(c.device);
   if (c.stream) (void)hipStreamSynchronize(c.stream);
   poisson_destroy(c);
+  tile_sort_destroy(c);
   void *ptrs[] = {c.x, c.y, c.w, c.h, c.area, c.kind, c.in_stage, c.npins, c.big, c.is_big, c.tx, c.ty,
                   c.net_start, c.pin_obj, c.pin_dx, c.pin_dy, c.group_begin, c.group_end, c.pin_net, c.large_nets, c.node_pin_start,
                   c.node_pins, c.pin_grad, c.bllx, c.burx, c.blly, c.bury, c.node_q, c.filler_q, c.term_map,

@@ -55,6 +55,21 @@ constexpr int kPartialTotal = 8 * kMaxPartials;
 
 struct Poisson;  // poisson.hip
 
+// Tile-sorted scatter state (density.hip).  Keys are kind * kTileKeyStride +
+// tile; kTileKeySentinel marks objects handled by the block-per-object path.
+constexpr uint32_t kTileKeyStride = 4096;
+constexpr uint32_t kTileKeySentinel = 3 * kTileKeyStride;
+constexpr int kTileKeyBits = 14;
+struct TileSort {
+  int tyc = 0, grid = 0;
+  uint32_t *keys[2] = {nullptr, nullptr};
+  float4 *vals[2] = {nullptr, nullptr};
+  int *ranges = nullptr, *nchunks = nullptr;
+  int4 *chunks = nullptr;
+  void *temp = nullptr;
+  size_t temp_bytes = 0;
+};
+
 struct Ctx {
   std::string err;
   int device = 0;
@@ -123,6 +138,7 @@ struct Ctx {
   double *partials = nullptr;
   unsigned int *tickets = nullptr;
   Poisson *poisson = nullptr;
+  TileSort tiles;
 
   Grid grid() const {
     Grid g;
@@ -149,6 +165,8 @@ void launch_scatter(Ctx &c);
 void launch_finalize_density(Ctx &c);
 void launch_gather_combine(Ctx &c, bool recompute_gather, bool lipschitz, const float2 *ref_now,
                            const float2 *ref_prev, const float2 *grad_prev);
+bool tile_sort_create(Ctx &c, std::string &err);
+void tile_sort_destroy(Ctx &c);
 // ---- wirelength.hip ----
 void launch_wirelength(Ctx &c, bool gradient);
 void launch_reduce_wirelength(Ctx &c, int lambda_update);
