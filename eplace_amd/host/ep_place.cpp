@@ -1,0 +1,167 @@
+// ep_place: the global-placement part of the reference's driver
+// (main/ePlace_main.cpp:35-223) over libeplace_amd.so.
+//
+//   ep_place -aux <design.aux> [-targetDensity 1.0] [-targetOverflow 0.10]
+//            [-loadpl <file.pl>] [-bktrk] [-LSE] [-bins <n>] [-nomGP]
+//            [-outputPath <dir>] [-dumpDesign <file>]
+//
+// Reads the Bookshelf design, runs mGP with EplaceNesterovOpt and writes
+// <outputPath>/<name>/<name>-eGP.pl (lower-left corners, as
+// PlaceDB::outputBookShelf does, placedb.cpp:1022-1060).  The stages the
+// library does not hold are reported and skipped: QP (the .pl or -loadpl
+// positions are the start, as with -noQP) and, for designs with macros,
+// mLG + FILLERONLY + cGP (as with -nomLG).  Logging goes to stdout.
+// -dumpDesign writes the parsed arrays (ep_design field order, raw
+// little-endian) to a file and exits before any device work.
+#include <sys/stat.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "bookshelf.hpp"
+#include "eplacer.hpp"
+
+using namespace eplace_amd;
+
+namespace {
+
+struct Args {
+  std::string aux, loadpl, out = ".", dump;
+  float td = 1.0f, tof = 0.10f;
+  bool bktrk = false, lse = false, no_mgp = false;
+  int bins = 0;
+};
+
+int usage(const char *prog) {
+  fprintf(stderr,
+          "usage: %s -aux <design.aux> [-targetDensity d] [-targetOverflow t] [-loadpl f.pl]\n"
+          "          [-bktrk] [-LSE] [-bins n] [-nomGP] [-outputPath dir] [-dumpDesign file]\n",
+          prog);
+  return 2;
+}
+
+bool parse(int argc, char **argv, Args &a) {
+  for (int i = 1; i < argc; ++i) {
+    std::string k = argv[i];
+    auto val = [&](std::string &dst) {
+      if (i + 1 >= argc) return false;
+      dst = argv[++i];
+      return true;
+    };
+    std::string v;
+    if (k == "-aux") {
+      if (!val(a.aux)) return false;
+    } else if (k == "-loadpl") {
+      if (!val(a.loadpl)) return false;
+    } else if (k == "-outputPath") {
+      if (!val(a.out)) return false;
+    } else if (k == "-dumpDesign") {
+      if (!val(a.dump)) return false;
+    } else if (k == "-targetDensity") {
+      if (!val(v)) return false;
+      a.td = (float)atof(v.c_str());
+    } else if (k == "-targetOverflow") {
+      if (!val(v)) return false;
+      a.tof = (float)atof(v.c_str());
+    } else if (k == "-bins") {
+      if (!val(v)) return false;
+      a.bins = atoi(v.c_str());
+    } else if (k == "-bktrk") {
+      a.bktrk = true;
+    } else if (k == "-LSE") {
+      a.lse = true;
+    } else if (k == "-nomGP") {
+      a.no_mgp = true;
+    } else if (k == "-noQP" || k == "-nomLG" || k == "-nocGP" || k == "-noLegal") {
+      // these stages are not run here; accepted for command-line compatibility
+    } else {
+      fprintf(stderr, "unknown option %s\n", k.c_str());
+      return false;
+    }
+  }
+  return !a.aux.empty();
+}
+
+void mkdirs(const std::string &path) {
+  std::string cur;
+  for (size_t i = 0; i <= path.size(); ++i) {
+    if (i == path.size() || path[i] == '/') {
+      if (!cur.empty()) mkdir(cur.c_str(), 0755);
+    }
+    if (i < path.size()) cur += path[i];
+  }
+}
+
+template <typename T>
+void put(FILE *f, const std::vector<T> &v) {
+  int64_t n = (int64_t)v.size();
+  fwrite(&n, sizeof n, 1, f);
+  if (n) fwrite(v.data(), sizeof(T), v.size(), f);
+}
+
+void dump_design(const BookshelfDesign &d, const std::string &path) {
+  FILE *f = fopen(path.c_str(), "wb");
+  if (!f) throw std::runtime_error("cannot write " + path);
+  put(f, d.node_w), put(f, d.node_h), put(f, d.node_x), put(f, d.node_y);
+  put(f, d.term_w), put(f, d.term_h), put(f, d.term_x), put(f, d.term_y), put(f, d.term_ni);
+  put(f, d.net_start), put(f, d.pin_obj), put(f, d.pin_dx), put(f, d.pin_dy);
+  put(f, d.row_bottom), put(f, d.row_height), put(f, d.row_x0), put(f, d.row_x1);
+  fclose(f);
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  Args a;
+  if (!parse(argc, argv, a)) return usage(argv[0]);
+  try {
+    BookshelfDesign d = read_bookshelf(a.aux);
+    if (!a.loadpl.empty()) load_pl(d, a.loadpl);
+    printf("design %s: %zu nodes, %zu terminals, %zu nets, %zu pins, %zu rows\n", d.name.c_str(),
+           d.node_w.size(), d.term_w.size(), d.net_start.size() - 1, d.pin_obj.size(), d.row_bottom.size());
+    if (!a.dump.empty()) {
+      dump_design(d, a.dump);
+      return 0;
+    }
+    printf("QP is not run: starting from the .pl positions\n");
+
+    ep_design view = d.view();
+    PlacerOptions o;
+    o.bins_x = o.bins_y = a.bins;
+    o.lse = a.lse;
+    EPlacer_2D placer(view, a.td, o);
+    placer.initialization();
+    ep_info info = placer.info();
+    printf("bins %dx%d, %lld fillers, target density %.3f\n", info.bins_x, info.bins_y, (long long)info.num_fillers, a.td);
+
+    if (!a.no_mgp) {
+      EplaceNesterovOpt<VECTOR_3D> opt(&placer, a.bktrk, a.tof);
+      opt.opt();
+      const ep_run_stats &s = opt.last();
+      printf("mGP finished: %d iterations, %d evaluations, overflow %.4f\n", s.iterations, s.evaluations,
+             s.final_overflow);
+      printf("Final HPWL: %.6e\n", s.final_hpwl);
+      printf("mGP time: %.4f s\n", s.seconds);
+    }
+    size_t macros = 0;
+    for (float h : d.node_h)  // PlaceDB::addNode (placedb.cpp:102-111)
+      macros += (double)(h - (float)view.common_row_height) > 1e-15;
+    if (macros > 0) printf("%zu macros: mLG, FILLERONLY and cGP are not run\n", macros);
+
+    std::vector<float> xy(2 * d.node_w.size());
+    std::vector<float> all(2 * (size_t)info.num_objects);
+    placer.check(ep_read(placer.context(), EP_ARR_POSITIONS, all.data(), (int64_t)all.size()));
+    std::copy(all.begin(), all.begin() + xy.size(), xy.begin());
+    std::string dir = a.out + "/" + d.name + "/";
+    mkdirs(dir);
+    std::string pl = dir + d.name + "-eGP.pl";
+    write_pl(d, xy, pl);
+    printf("wrote %s\n", pl.c_str());
+  } catch (const std::exception &e) {
+    fprintf(stderr, "ep_place: %s\n", e.what());
+    return 1;
+  }
+  return 0;
+}

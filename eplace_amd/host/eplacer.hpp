@@ -1,0 +1,169 @@
+// C++ host mirror of the reference's global-placement API over the C ABI.
+//
+// EPlacer_2D keeps the method names, call order and stage semantics of the
+// reference's EPlacer_2D (EPlace/eplace.h:95-200); FirstOrderOptimizer and
+// EplaceNesterovOpt keep opt() (Optimization/opt.hpp:18-36,
+// nesterov.hpp:59-415).  The optimiser loop itself runs device-resident
+// through ep_nesterov_run; positions and gradients are std::vector<VECTOR_3D>
+// in the reference's stage order with z = 0.  Errors throw
+// std::runtime_error carrying ep_last_error(); nothing calls exit().
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/eplace_amd.h"
+
+namespace eplace_amd {
+
+struct VECTOR_3D {
+  float x = 0, y = 0, z = 0;
+};
+
+enum PLACEMENT_STAGE { mGP = EP_STAGE_MGP, FILLERONLY = EP_STAGE_FILLERONLY, cGP = EP_STAGE_CGP };
+
+struct PlacerOptions {
+  int bins_x = 0, bins_y = 0;  // 0: the reference's bin rule
+  bool lse = false;            // -LSE
+  bool true_exp = false;       // exp instead of (1+a/1024)^1024
+  uint32_t filler_seed = 1;
+  int device = 0;
+};
+
+class EPlacer_2D {
+ public:
+  EPlacer_2D(const ep_design &design, float targetDensity, const PlacerOptions &o = PlacerOptions()) {
+    ep_params p{};
+    p.target_density = targetDensity;
+    p.bins_x = o.bins_x;
+    p.bins_y = o.bins_y;
+    p.wirelength_model = o.lse ? 1 : 0;
+    p.exp_mode = o.true_exp ? 1 : 0;
+    p.filler_seed = o.filler_seed;
+    p.filler_xy = nullptr;
+    p.filler_count = -1;
+    p.poisson_mode = 0;
+    p.device = o.device;
+    int rc = ep_create(&design, &p, &ctx_);
+    if (rc != EP_OK) {
+      std::string msg = ctx_ ? ep_last_error(ctx_) : "ep_create failed";
+      if (ctx_) ep_destroy(ctx_);
+      ctx_ = nullptr;
+      throw std::runtime_error("ep_create: " + msg);
+    }
+    targetDensity_ = targetDensity;
+  }
+  ~EPlacer_2D() {
+    if (ctx_) ep_destroy(ctx_);
+  }
+  EPlacer_2D(const EPlacer_2D &) = delete;
+  EPlacer_2D &operator=(const EPlacer_2D &) = delete;
+
+  // eplace.cpp:39-50 (fillers and bins are built by the constructor)
+  void initialization() {
+    totalGradientUpdate();
+    penaltyFactorInitilization();
+  }
+  void totalGradientUpdate() { check(ep_total_gradient_update(ctx_)); }
+  std::vector<VECTOR_3D> getGradient() { return pairs(&ep_get_gradient); }
+  std::vector<VECTOR_3D> getPosition() { return pairs(&ep_get_position); }
+  void setPosition(const std::vector<VECTOR_3D> &v) {
+    std::vector<float> xy(2 * v.size());
+    for (size_t i = 0; i < v.size(); ++i) {
+      xy[2 * i] = v[i].x;
+      xy[2 * i + 1] = v[i].y;
+    }
+    check(ep_set_position(ctx_, xy.data(), (int64_t)v.size()));
+  }
+  void penaltyFactorInitilization() { check(ep_penalty_factor_init(ctx_)); }
+  void updatePenaltyFactor() { check(ep_update_penalty_factor(ctx_)); }
+  void switch2FillerOnly() {
+    check(ep_switch_to_filler_only(ctx_));
+    placementStage = FILLERONLY;
+  }
+  void switch2cGP() {
+    check(ep_switch_to_cgp(ctx_, mGPIterationCount));
+    placementStage = cGP;
+  }
+  ep_stats stats() {
+    ep_stats s{};
+    check(ep_get_stats(ctx_, &s));
+    return s;
+  }
+  double calcHPWL() {
+    double v = 0;
+    check(ep_hpwl(ctx_, &v));
+    return v;
+  }
+  std::string showInfo() {
+    ep_stats s = stats();
+    return "Overflow: " + std::to_string(s.overflow) + "\npenalty factor: " + std::to_string(s.lambda) +
+           "\nHPWL: " + std::to_string(s.last_hpwl) + "\n";
+  }
+  ep_info info() {
+    ep_info i{};
+    check(ep_get_info(ctx_, &i));
+    return i;
+  }
+  ep_ctx *context() { return ctx_; }
+  void check(int rc) {
+    if (rc != EP_OK) throw std::runtime_error(std::string("eplace_amd: ") + ep_last_error(ctx_));
+  }
+
+  int placementStage = mGP;
+  int mGPIterationCount = 0;
+
+ private:
+  std::vector<VECTOR_3D> pairs(int (*fn)(ep_ctx *, float *, int64_t)) {
+    int64_t n = ep_stage_size(ctx_);
+    std::vector<float> xy(2 * (size_t)n);
+    check(fn(ctx_, xy.data(), n));
+    std::vector<VECTOR_3D> v((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+      v[i].x = xy[2 * i];
+      v[i].y = xy[2 * i + 1];
+    }
+    return v;
+  }
+  ep_ctx *ctx_ = nullptr;
+  float targetDensity_ = 1.f;
+};
+
+template <typename T>
+class FirstOrderOptimizer {
+ public:
+  virtual ~FirstOrderOptimizer() {}
+  virtual void opt() = 0;
+};
+
+// Nesterov with vanilla (default) or backtracking (-bktrk) step length.
+template <typename T>
+class EplaceNesterovOpt : public FirstOrderOptimizer<T> {
+ public:
+  explicit EplaceNesterovOpt(EPlacer_2D *placer, bool bktrk = false, float targetOverflow = 0.10f)
+      : placer_(placer), bktrk_(bktrk), targetOverflow_(targetOverflow) {}
+  void opt() override {
+    ep_opt_params p{};
+    p.method = bktrk_ ? 1 : 0;
+    p.target_overflow = targetOverflow_;
+    p.max_iterations = 1000;  // MAX_ITERATION (nesterov.hpp:36)
+    p.max_backtracks = 30;
+    p.sync_every = 16;
+    p.log_capacity = 0;
+    p.log = nullptr;
+    p.phase_seconds = nullptr;
+    placer_->check(ep_nesterov_run(placer_->context(), &p, &last_));
+    if (placer_->placementStage == mGP) placer_->mGPIterationCount = last_.iterations;
+  }
+  const ep_run_stats &last() const { return last_; }
+
+ private:
+  EPlacer_2D *placer_;
+  bool bktrk_;
+  float targetOverflow_;
+  ep_run_stats last_{};
+};
+
+}  // namespace eplace_amd
