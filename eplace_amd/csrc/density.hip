@@ -79,29 +79,6 @@ __device__ __forceinline__ unsigned long long fixed_contribution(float s, double
   return (unsigned long long)__double2ll_rn((double)c * scale);
 }
 
-__global__ void __launch_bounds__(kBlock) scatter_kernel(int O, const float *__restrict__ x,
-                                                          const float *__restrict__ y,
-                                                          const float *__restrict__ w,
-                                                          const float *__restrict__ h,
-                                                          const uint8_t *__restrict__ kind,
-                                                          const uint8_t *__restrict__ is_big, Grid g,
-                                                          double scale, unsigned long long *node_q,
-                                                          unsigned long long *filler_q) {
-  int o = blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= O || is_big[o]) return;
-  uint8_t k = kind[o];
-  Foot f = footprint(x[o], y[o], w[o], h[o], g);
-  float s = f.sx * f.sy;
-  if (k == KIND_MACRO) s = s * g.td;
-  unsigned long long *dst = k == KIND_FILLER ? filler_q : node_q;
-  for (int i = f.i0; i <= f.i1; ++i)
-    for (int j = f.j0; j <= f.j1; ++j) {
-      double ov = bin_overlap(g, i, j, f);
-      if (ov == 0) continue;
-      atomicAdd(dst + (size_t)i * g.ny + j, fixed_contribution(s, ov, scale));
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Tile-sorted scatter.  Hot bins (clustered placements put hundreds of
 // contributions on one bin) serialise HBM/L2 atomics, so objects are sorted
@@ -114,8 +91,7 @@ __global__ void __launch_bounds__(kBlock) scatter_kernel(int O, const float *__r
 // ---------------------------------------------------------------------------
 constexpr int kTile = 32, kTileHalo = 3, kTileWin = kTile + kTileHalo, kTileChunk = 1024;
 
-__global__ void __launch_bounds__(kBlock) tile_key_kernel(int O, const float *__restrict__ x,
-                                                           const float *__restrict__ y,
+__global__ void __launch_bounds__(kBlock) tile_key_kernel(int O, const float2 *__restrict__ pos,
                                                            const float *__restrict__ w,
                                                            const float *__restrict__ h,
                                                            const uint8_t *__restrict__ kind,
@@ -125,7 +101,8 @@ __global__ void __launch_bounds__(kBlock) tile_key_kernel(int O, const float *__
   int o = blockIdx.x * blockDim.x + threadIdx.x;
   if (o < nranges) ranges[o] = 0;
   if (o >= O) return;
-  float cx = x[o], cy = y[o], ww = w[o], hh = h[o];
+  const float2 p = pos[o];
+  float cx = p.x, cy = p.y, ww = w[o], hh = h[o];
   uint32_t key = kTileKeySentinel;
   if (!is_big[o]) {
     Foot f = footprint(cx, cy, ww, hh, g);
@@ -217,8 +194,7 @@ __global__ void __launch_bounds__(kBlock) tile_scatter_kernel(const int4 *__rest
 
 // Objects covering many bins (macros): one block each, threads over bins.
 __global__ void __launch_bounds__(kBlock) scatter_big_kernel(const int32_t *__restrict__ big,
-                                                              const float *__restrict__ x,
-                                                              const float *__restrict__ y,
+                                                              const float2 *__restrict__ pos,
                                                               const float *__restrict__ w,
                                                               const float *__restrict__ h,
                                                               const uint8_t *__restrict__ kind, Grid g,
@@ -226,7 +202,7 @@ __global__ void __launch_bounds__(kBlock) scatter_big_kernel(const int32_t *__re
                                                               unsigned long long *filler_q) {
   int o = big[blockIdx.x];
   uint8_t k = kind[o];
-  Foot f = footprint(x[o], y[o], w[o], h[o], g);
+  Foot f = footprint(pos[o].x, pos[o].y, w[o], h[o], g);
   float s = f.sx * f.sy;
   if (k == KIND_MACRO) s = s * g.td;
   unsigned long long *dst = k == KIND_FILLER ? filler_q : node_q;
@@ -311,8 +287,7 @@ __device__ __forceinline__ float2 gather_one(const Foot &f, const Grid &g, int l
 }
 
 __global__ void __launch_bounds__(kBlock) gather_big_kernel(const int32_t *__restrict__ big,
-                                                             const float *__restrict__ x,
-                                                             const float *__restrict__ y,
+                                                             const float2 *__restrict__ pos,
                                                              const float *__restrict__ w,
                                                              const float *__restrict__ h, Grid g, int ldp,
                                                              const float *__restrict__ ex,
@@ -320,7 +295,7 @@ __global__ void __launch_bounds__(kBlock) gather_big_kernel(const int32_t *__res
                                                              const DevScalars *sc) {
   if (sc->done) return;
   int o = big[blockIdx.x];
-  Foot f = footprint(x[o], y[o], w[o], h[o], g);
+  Foot f = footprint(pos[o].x, pos[o].y, w[o], h[o], g);
   int ni = f.i1 - f.i0 + 1, nj = f.j1 - f.j0 + 1;
   double gx = 0, gy = 0;
   if (ni > 0 && nj > 0) {
@@ -340,9 +315,11 @@ __global__ void __launch_bounds__(kBlock) gather_big_kernel(const int32_t *__res
 // node's pins in modulePins order (eplace.cpp:531-549), preconditioned
 // combine (eplace.cpp:694-737) and, optionally, the Lipschitz partial sums
 // |g - g_prev|^2 and |y - y_prev|^2 over the stage's objects
-// (optutils.hpp:19-35).
+// (optutils.hpp:19-35).  Blocks walk the objects XCD-contiguously: with
+// nets stored by their lowest node, the pin gradients a range of nodes
+// reads lie in a narrow band of the pin array.
 __global__ void __launch_bounds__(kBlock) gather_combine_kernel(
-    int O, int N, const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ w,
+    int O, int N, const float2 *__restrict__ pos, const float *__restrict__ w,
     const float *__restrict__ h, const uint8_t *__restrict__ kind, const uint8_t *__restrict__ is_big,
     const int32_t *__restrict__ npins, const float *__restrict__ area, Grid g, int ldp,
     const float *__restrict__ ex, const float *__restrict__ ey, const int32_t *__restrict__ node_pin_start,
@@ -351,13 +328,14 @@ __global__ void __launch_bounds__(kBlock) gather_combine_kernel(
     const float2 *__restrict__ ref_now, const float2 *__restrict__ ref_prev,
     const float2 *__restrict__ grad_prev, double *partials, const DevScalars *sc) {
   if (sc->done) return;
-  int o = blockIdx.x * blockDim.x + threadIdx.x;
+  int o = xcd_block() * blockDim.x + threadIdx.x;
   double ln = 0, ld = 0;
   if (o < O) {
     uint8_t k = kind[o];
     float2 d;
     if (recompute && !is_big[o]) {
-      Foot f = footprint(x[o], y[o], w[o], h[o], g);
+      const float2 p = pos[o];
+      Foot f = footprint(p.x, p.y, w[o], h[o], g);
       float gx = 0, gy = 0;
       for (int i = f.i0; i <= f.i1; ++i)
         for (int j = f.j0; j <= f.j1; ++j) {
@@ -397,7 +375,7 @@ __global__ void __launch_bounds__(kBlock) gather_combine_kernel(
     gtot[o] = t;
     if (lipschitz && in_stage[o]) {
       float2 gp = grad_prev[o];
-      float2 rn = ref_now ? ref_now[o] : make_float2(x[o], y[o]);
+      float2 rn = ref_now ? ref_now[o] : pos[o];
       float2 rp = ref_prev[o];
       double dgx = (double)t.x - gp.x, dgy = (double)t.y - gp.y;
       double dxx = (double)rn.x - rp.x, dxy = (double)rn.y - rp.y;
@@ -463,7 +441,7 @@ void launch_scatter(Ctx &c) {
   TileSort &t = c.tiles;
   const int nranges = 2 * kTileKeySentinel;
   hipLaunchKernelGGL(tile_key_kernel, dim3(blocks_for(std::max(c.O, nranges))), dim3(kBlock), 0, c.stream, c.O,
-                     c.x, c.y, c.w, c.h, c.kind, c.is_big, g, t.tyc, t.keys[0], t.vals[0], t.ranges, nranges);
+                     c.pos, c.w, c.h, c.kind, c.is_big, g, t.tyc, t.keys[0], t.vals[0], t.ranges, nranges);
   hipcub::DoubleBuffer<uint32_t> kb(t.keys[0], t.keys[1]);
   hipcub::DoubleBuffer<float4> vb(t.vals[0], t.vals[1]);
   size_t bytes = t.temp_bytes;
@@ -475,7 +453,7 @@ void launch_scatter(Ctx &c) {
   hipLaunchKernelGGL(tile_scatter_kernel, dim3(t.grid), dim3(kBlock), 0, c.stream, t.chunks, t.nchunks,
                      vb.Current(), g, t.tyc, scale, c.node_q, c.filler_q);
   if (c.num_big)
-    hipLaunchKernelGGL(scatter_big_kernel, dim3(c.num_big), dim3(kBlock), 0, c.stream, c.big, c.x, c.y, c.w,
+    hipLaunchKernelGGL(scatter_big_kernel, dim3(c.num_big), dim3(kBlock), 0, c.stream, c.big, c.pos, c.w,
                        c.h, c.kind, g, scale, c.node_q, c.filler_q);
 }
 
@@ -532,10 +510,10 @@ void launch_gather_combine(Ctx &c, bool recompute, bool lipschitz, const float2 
   Grid g = c.grid();
   int ldp = poisson_ld(c);
   if (recompute && c.num_big)
-    hipLaunchKernelGGL(gather_big_kernel, dim3(c.num_big), dim3(kBlock), 0, c.stream, c.big, c.x, c.y, c.w, c.h,
+    hipLaunchKernelGGL(gather_big_kernel, dim3(c.num_big), dim3(kBlock), 0, c.stream, c.big, c.pos, c.w, c.h,
                        g, ldp, c.ex, c.ey, c.gd, c.sc);
-  hipLaunchKernelGGL(gather_combine_kernel, dim3(blocks_for(c.O)), dim3(kBlock), 0, c.stream, c.O, c.N, c.x,
-                     c.y, c.w, c.h, c.kind, c.is_big, c.npins, c.area, g, ldp, c.ex, c.ey, c.node_pin_start,
+  hipLaunchKernelGGL(gather_combine_kernel, dim3(object_blocks(c.O)), dim3(kBlock), 0, c.stream, c.O, c.N,
+                     c.pos, c.w, c.h, c.kind, c.is_big, c.npins, c.area, g, ldp, c.ex, c.ey, c.node_pin_start,
                      c.node_pins, c.pin_grad, c.gd, c.gwl, c.gtot, recompute ? 1 : 0, lipschitz ? 1 : 0,
                      c.in_stage, ref_now, ref_prev, grad_prev, c.partials, c.sc);
 }

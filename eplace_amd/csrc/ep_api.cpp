@@ -357,14 +357,49 @@ int create(Ctx &c, const ep_design *d, const ep_params *p) {
     }
   }
   c.num_big = (int)big.size();
-  // ---- nets: degree > 64 get a wave each; runs of consecutive smaller nets
-  // are packed into groups of at most 64 pins (one wave, one pin per lane) ----
-  std::vector<int32_t> large, gbeg, gend, ns32(c.E + 1), pin_net(std::max<int64_t>(c.P, 1));
+  // ---- nets, stored by their lowest movable node (stable), so that runs of
+  // consecutive nets read the centres of a narrow band of objects; pins keep
+  // their order inside a net.  Objects are numbered nodes, fillers, then
+  // terminals (pin_obj of terminal t becomes O + t). ----
+  std::vector<int32_t> order(c.E);
+  {
+    std::vector<int32_t> key(c.E), cnt(c.N + 2, 0);
+    for (int e = 0; e < c.E; ++e) {
+      int32_t k = c.N;
+      for (int64_t q = d->net_start[e]; q < d->net_start[e + 1]; ++q)
+        if (d->pin_obj[q] < c.N) k = std::min(k, d->pin_obj[q]);
+      key[e] = k;
+      cnt[k + 1]++;
+    }
+    for (int k = 0; k <= c.N; ++k) cnt[k + 1] += cnt[k];
+    for (int e = 0; e < c.E; ++e) order[cnt[key[e]]++] = e;
+  }
+  std::vector<int32_t> ns32(c.E + 1), pobj(std::max<int64_t>(c.P, 1)), pin_net(std::max<int64_t>(c.P, 1));
+  std::vector<float> pdx(std::max<int64_t>(c.P, 1)), pdy(std::max<int64_t>(c.P, 1));
+  c.h_pin_of.assign(c.P, 0);
+  {
+    int32_t q2 = 0;
+    for (int e2 = 0; e2 < c.E; ++e2) {
+      int e = order[e2];
+      ns32[e2] = q2;
+      for (int64_t q = d->net_start[e]; q < d->net_start[e + 1]; ++q, ++q2) {
+        int32_t o = d->pin_obj[q];
+        pobj[q2] = o < c.N ? o : c.O + (o - c.N);
+        pdx[q2] = d->pin_dx[q];
+        pdy[q2] = d->pin_dy[q];
+        pin_net[q2] = e2;
+        c.h_pin_of[q] = q2;
+      }
+    }
+    ns32[c.E] = q2;
+  }
+  // degree > 64 get a wave each; runs of consecutive smaller nets are packed
+  // into groups of at most 64 pins (one wave, one pin per lane)
+  std::vector<int32_t> large, gbeg, gend;
   {
     int32_t cur_b = -1, cur_e = -1;
     for (int e = 0; e < c.E; ++e) {
-      int32_t s0 = (int32_t)d->net_start[e], s1 = (int32_t)d->net_start[e + 1];
-      for (int32_t q = s0; q < s1; ++q) pin_net[q] = e;
+      int32_t s0 = ns32[e], s1 = ns32[e + 1];
       if (s1 == s0) continue;
       if (s1 - s0 > kSmallNetMaxDegree) {
         large.push_back(e);
@@ -377,30 +412,29 @@ int create(Ctx &c, const ep_design *d, const ep_params *p) {
     }
     if (cur_b >= 0) { gbeg.push_back(cur_b); gend.push_back(cur_e); }
   }
-  for (int e = 0; e <= c.E; ++e) ns32[e] = (int32_t)d->net_start[e];
   c.num_groups = (int)gbeg.size();
   c.num_large = (int)large.size();
-  // ---- node -> pins CSR in modulePins (file) order ----
+  // ---- node -> pins CSR in modulePins (file) order, as device pin indices ----
   std::vector<int32_t> nps(c.N + 1, 0), npl(std::max<int64_t>(c.P, 1));
   for (int i = 0; i < c.N; ++i) nps[i + 1] = nps[i] + s.npins[i];
   {
     std::vector<int32_t> fill(nps.begin(), nps.end() - 1);
     for (int64_t q = 0; q < c.P; ++q)
-      if (d->pin_obj[q] < c.N) npl[fill[d->pin_obj[q]]++] = (int32_t)q;
+      if (d->pin_obj[q] < c.N) npl[fill[d->pin_obj[q]]++] = c.h_pin_of[q];
   }
   c.h_w = s.w;
   c.h_h = s.h;
   c.h_kind = s.kind;
   // ---- device buffers ----
-  std::vector<float> tx(d->term_x, d->term_x + c.NT), ty(d->term_y, d->term_y + c.NT);
-  bool ok = dev_upload(c, c.x, s.x.data(), c.O) && dev_upload(c, c.y, s.y.data(), c.O) &&
-            dev_upload(c, c.w, s.w.data(), c.O) && dev_upload(c, c.h, s.h.data(), c.O) &&
+  std::vector<float2> pos((size_t)c.O + c.NT);
+  for (int o = 0; o < c.O; ++o) pos[o] = make_float2(s.x[o], s.y[o]);
+  for (int t = 0; t < c.NT; ++t) pos[(size_t)c.O + t] = make_float2(d->term_x[t], d->term_y[t]);
+  bool ok = dev_upload(c, c.pos, pos.data(), pos.size()) && dev_upload(c, c.w, s.w.data(), c.O) && dev_upload(c, c.h, s.h.data(), c.O) &&
             dev_upload(c, c.area, s.area.data(), c.O) && dev_upload(c, c.kind, s.kind.data(), c.O) &&
             dev_zero(c, c.in_stage, c.O) && dev_upload(c, c.npins, s.npins.data(), c.O) &&
             dev_upload(c, c.big, big.data(), big.size()) && dev_upload(c, c.is_big, is_big.data(), c.O) &&
-            dev_upload(c, c.tx, tx.data(), c.NT) && dev_upload(c, c.ty, ty.data(), c.NT) &&
-            dev_upload(c, c.net_start, ns32.data(), c.E + 1) && dev_upload(c, c.pin_obj, d->pin_obj, c.P) &&
-            dev_upload(c, c.pin_dx, d->pin_dx, c.P) && dev_upload(c, c.pin_dy, d->pin_dy, c.P) &&
+            dev_upload(c, c.net_start, ns32.data(), c.E + 1) && dev_upload(c, c.pin_obj, pobj.data(), c.P) &&
+            dev_upload(c, c.pin_dx, pdx.data(), c.P) && dev_upload(c, c.pin_dy, pdy.data(), c.P) &&
             dev_upload(c, c.group_begin, gbeg.data(), gbeg.size()) &&
             dev_upload(c, c.group_end, gend.data(), gend.size()) &&
             dev_upload(c, c.pin_net, pin_net.data(), pin_net.size()) &&
@@ -437,7 +471,7 @@ void destroy(Ctx &c) {
   if (c.stream) (void)hipStreamSynchronize(c.stream);
   poisson_destroy(c);
   tile_sort_destroy(c);
-  void *ptrs[] = {c.x, c.y, c.w, c.h, c.area, c.kind, c.in_stage, c.npins, c.big, c.is_big, c.tx, c.ty,
+  void *ptrs[] = {c.pos, c.w, c.h, c.area, c.kind, c.in_stage, c.npins, c.big, c.is_big,
                   c.net_start, c.pin_obj, c.pin_dx, c.pin_dy, c.group_begin, c.group_end, c.pin_net, c.large_nets, c.node_pin_start,
                   c.node_pins, c.pin_grad, c.bllx, c.burx, c.blly, c.bury, c.node_q, c.filler_q, c.term_map,
                   c.base_map, c.rho, c.phi, c.ex, c.ey, c.gd, c.gwl, c.gtot, c.cur_main, c.cur_ref, c.last_ref,
@@ -509,23 +543,13 @@ int hpwl_pass(Ctx &c, int lambda_update) {
   return sync(c);
 }
 
-// Stage-ordered float pairs from an [O] float2 (or x/y) device array.
-int gather_stage(Ctx &c, const float *dx, const float *dy, const float2 *d2, float *out, int64_t n) {
+// Stage-ordered float pairs from an [O] float2 device array.
+int gather_stage(Ctx &c, const float2 *d2, float *out, int64_t n) {
   if (n != (int64_t)c.h_stage_idx.size()) return c.err = "size does not match the stage", EP_ERR_ARG;
   int rc = sync(c);
   if (rc) return rc;
   std::vector<float> a((size_t)c.O * 2);
-  if (d2) {
-    EP_HIP(hipMemcpy(a.data(), d2, (size_t)c.O * sizeof(float2), hipMemcpyDeviceToHost));
-  } else {
-    std::vector<float> bx(c.O), by(c.O);
-    EP_HIP(hipMemcpy(bx.data(), dx, (size_t)c.O * 4, hipMemcpyDeviceToHost));
-    EP_HIP(hipMemcpy(by.data(), dy, (size_t)c.O * 4, hipMemcpyDeviceToHost));
-    for (int o = 0; o < c.O; ++o) {
-      a[2 * o] = bx[o];
-      a[2 * o + 1] = by[o];
-    }
-  }
+  EP_HIP(hipMemcpy(a.data(), d2, (size_t)c.O * sizeof(float2), hipMemcpyDeviceToHost));
   for (int64_t k = 0; k < n; ++k) {
     int o = c.h_stage_idx[k];
     out[2 * k] = a[2 * o];
@@ -771,7 +795,7 @@ int64_t ep_stage_size(ep_ctx *x) { return x ? (int64_t)x->c.h_stage_idx.size() :
 
 int ep_get_position(ep_ctx *x, float *xy, int64_t n) {
   if (!x || (!xy && n)) return EP_ERR_ARG;
-  return ep::gather_stage(x->c, x->c.x, x->c.y, nullptr, xy, n);
+  return ep::gather_stage(x->c, x->c.pos, xy, n);
 }
 
 int ep_set_position(ep_ctx *x, const float *xy, int64_t n) {
@@ -781,7 +805,7 @@ int ep_set_position(ep_ctx *x, const float *xy, int64_t n) {
 
 int ep_get_gradient(ep_ctx *x, float *xy, int64_t n) {
   if (!x || (!xy && n)) return EP_ERR_ARG;
-  return ep::gather_stage(x->c, nullptr, nullptr, x->c.gtot, xy, n);
+  return ep::gather_stage(x->c, x->c.gtot, xy, n);
 }
 
 int ep_penalty_factor_init(ep_ctx *x) {
@@ -803,12 +827,10 @@ int ep_switch_to_filler_only(ep_ctx *x) {
   Ctx &c = x->c;
   int rc = ep::sync(c);
   if (rc) return rc;
-  std::vector<float> hx(c.O), hy(c.O);
-  EP_HIP(hipMemcpy(hx.data(), c.x, (size_t)c.O * 4, hipMemcpyDeviceToHost));
-  EP_HIP(hipMemcpy(hy.data(), c.y, (size_t)c.O * 4, hipMemcpyDeviceToHost));
-  for (int o = c.N; o < c.O; ++o) ep::random_filler(c, c.h_w[o], c.h_h[o], hx[o], hy[o]);
-  EP_HIP(hipMemcpy(c.x, hx.data(), (size_t)c.O * 4, hipMemcpyHostToDevice));
-  EP_HIP(hipMemcpy(c.y, hy.data(), (size_t)c.O * 4, hipMemcpyHostToDevice));
+  std::vector<float2> p(c.O);
+  EP_HIP(hipMemcpy(p.data(), c.pos, (size_t)c.O * sizeof(float2), hipMemcpyDeviceToHost));
+  for (int o = c.N; o < c.O; ++o) ep::random_filler(c, c.h_w[o], c.h_h[o], p[o].x, p[o].y);
+  EP_HIP(hipMemcpy(c.pos, p.data(), (size_t)c.O * sizeof(float2), hipMemcpyHostToDevice));
   c.eval_valid = false;
   c.stage = EP_STAGE_FILLERONLY;
   ep::rebuild_stage(c);
@@ -908,17 +930,7 @@ int ep_read(ep_ctx *x, int32_t which, void *host, int64_t count) {
     return EP_OK;
   };
   switch (which) {
-    case EP_ARR_POSITIONS: {
-      if (count != 2 * (int64_t)c.O) return c.err = "count mismatch", EP_ERR_ARG;
-      std::vector<float> a(c.O), b(c.O);
-      EP_HIP(hipMemcpy(a.data(), c.x, (size_t)c.O * 4, hipMemcpyDeviceToHost));
-      EP_HIP(hipMemcpy(b.data(), c.y, (size_t)c.O * 4, hipMemcpyDeviceToHost));
-      for (int o = 0; o < c.O; ++o) {
-        ((float *)host)[2 * o] = a[o];
-        ((float *)host)[2 * o + 1] = b[o];
-      }
-      return EP_OK;
-    }
+    case EP_ARR_POSITIONS: return plain(c.pos, 2 * (int64_t)c.O, 4);
     case EP_ARR_NODE_Q: return plain(c.node_q, nb, 8);
     case EP_ARR_FILLER_Q: return plain(c.filler_q, nb, 8);
     case EP_ARR_RHO: return map_down(c.rho);
@@ -934,7 +946,13 @@ int ep_read(ep_ctx *x, int32_t which, void *host, int64_t count) {
     case EP_ARR_GRAD_TOTAL: return plain(c.gtot, 2 * (int64_t)c.O, 4);
     case EP_ARR_TERM_MAP: return plain(c.term_map, nb, 4);
     case EP_ARR_BASE_MAP: return plain(c.base_map, nb, 4);
-    case EP_ARR_PIN_GRAD: return plain(c.pin_grad, 2 * c.P, 4);
+    case EP_ARR_PIN_GRAD: {  // back to design pin order
+      if (count != 2 * c.P) return c.err = "count mismatch", EP_ERR_ARG;
+      std::vector<float2> g(std::max<int64_t>(c.P, 1));
+      EP_HIP(hipMemcpy(g.data(), c.pin_grad, (size_t)c.P * sizeof(float2), hipMemcpyDeviceToHost));
+      for (int64_t q = 0; q < c.P; ++q) ((float2 *)host)[q] = g[c.h_pin_of[q]];
+      return EP_OK;
+    }
     case EP_ARR_MAIN: return plain(c.cur_main, 2 * (int64_t)c.O, 4);
   }
   c.err = "unknown array";
@@ -958,13 +976,7 @@ int ep_write(ep_ctx *x, int32_t which, const void *host, int64_t count) {
   switch (which) {
     case EP_ARR_POSITIONS: {
       if (count != 2 * (int64_t)c.O) return c.err = "count mismatch", EP_ERR_ARG;
-      std::vector<float> a(c.O), b(c.O);
-      for (int o = 0; o < c.O; ++o) {
-        a[o] = ((const float *)host)[2 * o];
-        b[o] = ((const float *)host)[2 * o + 1];
-      }
-      EP_HIP(hipMemcpy(c.x, a.data(), (size_t)c.O * 4, hipMemcpyHostToDevice));
-      EP_HIP(hipMemcpy(c.y, b.data(), (size_t)c.O * 4, hipMemcpyHostToDevice));
+      EP_HIP(hipMemcpy(c.pos, host, (size_t)c.O * sizeof(float2), hipMemcpyHostToDevice));
       c.eval_valid = false;
       return EP_OK;
     }

@@ -17,6 +17,24 @@ constexpr double kEps = 1.0e-15;  // global.h:39
 constexpr int kBlock = 256;
 constexpr int kSmallNetMaxDegree = 64;  // nets above this get a wave each; the rest are packed 64 pins per wave
 constexpr int kBigObjectBins = 16;      // objects that may cover more bins get a block each
+constexpr int kXcds = 8;                // MI355X: workgroups go to the 8 XCDs round-robin
+
+// Renumbers blocks so that the blocks of each XCD cover one contiguous
+// 1/8 of the grid (block b runs on XCD b % 8).  Work ranges that are
+// contiguous in memory then stay inside one XCD's L2.  Grids that are not a
+// multiple of 8 keep their numbering.
+__device__ __forceinline__ int xcd_block() {
+  const int b = blockIdx.x, n = gridDim.x;
+  if (n % kXcds) return b;
+  return (b % kXcds) * (n / kXcds) + b / kXcds;
+}
+
+// Blocks of kBlock threads covering n items, rounded up to a multiple of
+// kXcds so that xcd_block() applies (the extra blocks find no work).
+inline int object_blocks(int64_t n) {
+  int64_t b = (n + kBlock - 1) / kBlock;
+  return (int)((b + kXcds - 1) / kXcds * kXcds);
+}
 
 // Bin grid, passed by value to the density kernels.  Edges are the
 // reference's float bin coordinates (eplace.cpp:292-314).
@@ -95,25 +113,26 @@ struct Ctx {
   std::vector<float> h_w, h_h;
   std::vector<uint8_t> h_kind;
   std::vector<int32_t> h_stage_idx;  // objects of the current stage, in stage order
+  std::vector<int32_t> h_pin_of;     // device pin index of each design pin (nets are reordered)
   float core_llx = 0, core_lly = 0, core_urx = 0, core_ury = 0, total_row_area = 0;
   float bsx = 0, bsy = 0;
   float std_area = 0, macro_area = 0, filler_w = 0, filler_h = 0;
   struct Rand *rng = nullptr;
   double lambda_host_init = 0;
 
+  // device: centres of the objects [O] followed by the terminals [NT]
+  float2 *pos = nullptr;
   // device: objects [O]
-  float *x = nullptr, *y = nullptr, *w = nullptr, *h = nullptr, *area = nullptr;
+  float *w = nullptr, *h = nullptr, *area = nullptr;
   uint8_t *kind = nullptr;
   uint8_t *in_stage = nullptr;
   int32_t *npins = nullptr;
   int32_t *big = nullptr;   // objects handled by the block-per-object kernels
   int num_big = 0;
   uint8_t *is_big = nullptr;
-  // terminals
-  float *tx = nullptr, *ty = nullptr;
-  // nets / pins
+  // nets / pins, nets ordered by their lowest movable node
   int32_t *net_start = nullptr;  // [E+1]
-  int32_t *pin_obj = nullptr;
+  int32_t *pin_obj = nullptr;    // index into pos: objects, then O + terminal
   float *pin_dx = nullptr, *pin_dy = nullptr;
   int32_t *large_nets = nullptr;
   int num_large = 0;

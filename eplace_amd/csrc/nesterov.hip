@@ -23,20 +23,12 @@ __device__ __forceinline__ float2 clamp_center(float x, float y, float w, float 
 }
 
 __global__ void set_positions_kernel(int64_t n, const float2 *__restrict__ xy, const int32_t *__restrict__ idx,
-                                     float *x, float *y, const float *__restrict__ w,
-                                     const float *__restrict__ h, Core k) {
+                                     float2 *pos, const float *__restrict__ w, const float *__restrict__ h,
+                                     Core k) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   int o = idx[t];
-  float2 c = clamp_center(xy[t].x, xy[t].y, w[o], h[o], k);
-  x[o] = c.x;
-  y[o] = c.y;
-}
-
-__global__ void positions_to_ref_kernel(int O, const float *__restrict__ x, const float *__restrict__ y,
-                                        float2 *dst) {
-  int o = blockIdx.x * blockDim.x + threadIdx.x;
-  if (o < O) dst[o] = make_float2(x[o], y[o]);
+  pos[o] = clamp_center(xy[t].x, xy[t].y, w[o], h[o], k);
 }
 
 __global__ void copy_float2_kernel(int64_t n, float2 *dst, const float2 *src) {
@@ -100,12 +92,12 @@ __global__ void __launch_bounds__(kBlock) accept_kernel(int nblocks, double *par
 __global__ void __launch_bounds__(kBlock) nesterov_move_kernel(
     int O, const uint8_t *__restrict__ in_stage, const float2 *ref, const float2 *__restrict__ grad,
     const float2 *main_in, float2 *main_out, float2 *ref_unclamped_out, float2 *save_ref, float2 *save_grad,
-    float *x, float *y, const float *__restrict__ w, const float *__restrict__ h, Core k, const DevScalars *sc) {
+    float2 *pos, const float *__restrict__ w, const float *__restrict__ h, Core k, const DevScalars *sc) {
   if (sc->done) return;
   int o = blockIdx.x * blockDim.x + threadIdx.x;
   if (o >= O || !in_stage[o]) return;
   float step = (float)sc->step, coef = (float)sc->coef;
-  float2 r = ref ? ref[o] : make_float2(x[o], y[o]);
+  float2 r = ref ? ref[o] : pos[o];
   float2 g = grad[o];
   float2 m = main_in[o];
   if (save_ref) save_ref[o] = r;
@@ -114,9 +106,7 @@ __global__ void __launch_bounds__(kBlock) nesterov_move_kernel(
   float nrx = npx + (npx - m.x) * coef, nry = npy + (npy - m.y) * coef;
   main_out[o] = make_float2(npx, npy);
   if (ref_unclamped_out) ref_unclamped_out[o] = make_float2(nrx, nry);
-  float2 c = clamp_center(nrx, nry, w[o], h[o], k);
-  x[o] = c.x;
-  y[o] = c.y;
+  pos[o] = clamp_center(nrx, nry, w[o], h[o], k);
 }
 
 // End of an iteration: commit a_{k+1}, advance the counter, evaluate the
@@ -157,13 +147,13 @@ static Core core_of(const Ctx &c) { return Core{c.core_llx, c.core_lly, c.core_u
 
 void launch_set_positions(Ctx &c, const float2 *xy, const int32_t *idx, int64_t n) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(set_positions_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, c.stream, n, xy, idx, c.x, c.y,
-                     c.w, c.h, core_of(c));
+  hipLaunchKernelGGL(set_positions_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, c.stream, n, xy, idx, c.pos, c.w,
+                     c.h, core_of(c));
 }
 
 void launch_positions_to_ref(Ctx &c, float2 *dst) {
-  hipLaunchKernelGGL(positions_to_ref_kernel, dim3(blocks_for(c.O)), dim3(kBlock), 0, c.stream, c.O, c.x, c.y,
-                     dst);
+  hipLaunchKernelGGL(copy_float2_kernel, dim3(blocks_for(c.O)), dim3(kBlock), 0, c.stream, (int64_t)c.O, dst,
+                     c.pos);
 }
 
 void launch_copy_float2(Ctx &c, float2 *dst, const float2 *src, int64_t n) {
@@ -171,17 +161,18 @@ void launch_copy_float2(Ctx &c, float2 *dst, const float2 *src, int64_t n) {
 }
 
 void launch_step_scalar(Ctx &c, int) {
-  hipLaunchKernelGGL(step_scalar_kernel, dim3(1), dim3(kBlock), 0, c.stream, blocks_for(c.O), c.partials, c.sc);
+  hipLaunchKernelGGL(step_scalar_kernel, dim3(1), dim3(kBlock), 0, c.stream, object_blocks(c.O), c.partials,
+                     c.sc);
 }
 
 void launch_accept(Ctx &c) {
-  hipLaunchKernelGGL(accept_kernel, dim3(1), dim3(kBlock), 0, c.stream, blocks_for(c.O), c.partials, c.sc);
+  hipLaunchKernelGGL(accept_kernel, dim3(1), dim3(kBlock), 0, c.stream, object_blocks(c.O), c.partials, c.sc);
 }
 
 void launch_nesterov_move_ex(Ctx &c, const float2 *ref, const float2 *grad, const float2 *main_in,
                              float2 *main_out, float2 *ref_unclamped_out, float2 *save_ref, float2 *save_grad) {
   hipLaunchKernelGGL(nesterov_move_kernel, dim3(blocks_for(c.O)), dim3(kBlock), 0, c.stream, c.O, c.in_stage,
-                     ref, grad, main_in, main_out, ref_unclamped_out, save_ref, save_grad, c.x, c.y, c.w, c.h,
+                     ref, grad, main_in, main_out, ref_unclamped_out, save_ref, save_grad, c.pos, c.w, c.h,
                      core_of(c), c.sc);
 }
 

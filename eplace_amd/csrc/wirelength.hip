@@ -18,17 +18,15 @@
 
 namespace ep {
 
+// Pin positions: centre of the pin's object or terminal (one packed 8-byte
+// read) plus the pin offset (objects.cpp:454-459).
 struct PinSrc {
   const int32_t *net_start, *pin_obj;
-  const float *pdx, *pdy, *x, *y, *tx, *ty;
-  int N;
-  __device__ __forceinline__ float px(int p) const {
-    int o = pin_obj[p];
-    return (o < N ? x[o] : tx[o - N]) + pdx[p];
-  }
-  __device__ __forceinline__ float py(int p) const {
-    int o = pin_obj[p];
-    return (o < N ? y[o] : ty[o - N]) + pdy[p];
+  const float *pdx, *pdy;
+  const float2 *pos;
+  __device__ __forceinline__ float2 at(int p) const {
+    const float2 c = pos[pin_obj[p]];
+    return make_float2(c.x + pdx[p], c.y + pdy[p]);
   }
 };
 
@@ -105,7 +103,8 @@ __global__ void __launch_bounds__(kBlock) wl_large_kernel(int num, const int32_t
     int s = src.net_start[e], n = src.net_start[e + 1];
     float xmax = -INFINITY, xmin = INFINITY, ymax = -INFINITY, ymin = INFINITY;
     for (int p = s + lane; p < n; p += 64) {
-      float px = src.px(p), py = src.py(p);
+      const float2 q = src.at(p);
+      const float px = q.x, py = q.y;
       xmax = fmaxf(xmax, px);
       xmin = fminf(xmin, px);
       ymax = fmaxf(ymax, py);
@@ -119,7 +118,8 @@ __global__ void __launch_bounds__(kBlock) wl_large_kernel(int num, const int32_t
     if (!GRAD) continue;
     NetAcc a = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int p = s + lane; p < n; p += 64) {
-      float px = src.px(p), py = src.py(p);
+      const float2 q = src.at(p);
+      const float px = q.x, py = q.y;
       float ep, em, fp, fm;
       pin_terms<MODEL, EXPM>(px, xmax, xmin, igx, ep, em);
       pin_terms<MODEL, EXPM>(py, ymax, ymin, igy, fp, fm);
@@ -144,7 +144,8 @@ __global__ void __launch_bounds__(kBlock) wl_large_kernel(int num, const int32_t
       wv += net_value<MODEL>(xmax, xmin, igx, a.dpx, a.npx, a.dmx, a.nmx) +
             net_value<MODEL>(ymax, ymin, igy, a.dpy, a.npy, a.dmy, a.nmy);
     for (int p = s + lane; p < n; p += 64) {
-      float px = src.px(p), py = src.py(p);
+      const float2 q = src.at(p);
+      const float px = q.x, py = q.y;
       float ep, em, fp, fm;
       pin_terms<MODEL, EXPM>(px, xmax, xmin, igx, ep, em);
       pin_terms<MODEL, EXPM>(py, ymax, ymin, igy, fp, fm);
@@ -177,7 +178,9 @@ __device__ __forceinline__ float seg_reduce(float v, const bool *same, Op op) {
 // lane (coalesced pin loads, no divergence on net degree).  Bound pins,
 // HPWL, the WA/LSE sums and the per-pin gradient come from segmented
 // shuffle reductions over each net's lanes; the net's head lane holds the
-// result and broadcasts it.
+// result and broadcasts it.  Each wave takes a contiguous run of groups and
+// the blocks are numbered XCD-contiguously, so with nets stored by their
+// lowest node one XCD reads the centres of a narrow band of objects.
 template <int MODEL, int EXPM, int GRAD>
 __global__ void __launch_bounds__(kBlock) wl_group_kernel(int num_groups, const int32_t *__restrict__ gbeg,
                                                            const int32_t *__restrict__ gend,
@@ -188,16 +191,20 @@ __global__ void __launch_bounds__(kBlock) wl_group_kernel(int num_groups, const 
   const float igx = (float)sc->inv_gamma_x, igy = (float)sc->inv_gamma_y;
   const int lane = threadIdx.x & 63;
   const int wpb = blockDim.x >> 6;
+  const int waves = gridDim.x * wpb, wid = xcd_block() * wpb + (threadIdx.x >> 6);
+  const int per = (num_groups + waves - 1) / waves;
+  const int gi0 = wid * per, gi1 = min(num_groups, gi0 + per);
   double hp = 0, wv = 0;
-  for (int gi = blockIdx.x * wpb + (threadIdx.x >> 6); gi < num_groups; gi += gridDim.x * wpb) {
+  for (int gi = gi0; gi < gi1; ++gi) {
     const int p0 = gbeg[gi], p1 = gend[gi];
     const int p = p0 + lane;
     const bool act = p < p1;
     int net = act ? pin_net[p] : -1 - lane;  // inactive lanes: unique segments
     float px = 0.f, py = 0.f;
     if (act) {
-      px = src.px(p);
-      py = src.py(p);
+      const float2 q = src.at(p);
+      px = q.x;
+      py = q.y;
     }
     bool same[6];
 #pragma unroll
@@ -268,7 +275,8 @@ __global__ void __launch_bounds__(kBlock) reduce_wl_kernel(int nslots, double *p
 static int g_small_blocks(const Ctx &c) {
   int wpb = kBlock / 64;
   int b = (c.num_groups + wpb - 1) / wpb;
-  return b < 1 ? 1 : (b > 8192 ? 8192 : b);
+  b = b < 1 ? 1 : (b > 8192 ? 8192 : b);
+  return (b + kXcds - 1) / kXcds * kXcds;
 }
 static int g_large_blocks(const Ctx &c) {
   int wpb = kBlock / 64;
@@ -278,7 +286,7 @@ static int g_large_blocks(const Ctx &c) {
 
 template <int MODEL, int EXPM, int GRAD>
 static void launch_wl_t(Ctx &c) {
-  PinSrc src{c.net_start, c.pin_obj, c.pin_dx, c.pin_dy, c.x, c.y, c.tx, c.ty, c.N};
+  PinSrc src{c.net_start, c.pin_obj, c.pin_dx, c.pin_dy, c.pos};
   int bs = g_small_blocks(c);
   hipLaunchKernelGGL((wl_group_kernel<MODEL, EXPM, GRAD>), dim3(bs), dim3(kBlock), 0, c.stream, c.num_groups,
                      c.group_begin, c.group_end, c.pin_net, src, c.sc, c.pin_grad, c.partials);
