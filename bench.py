@@ -30,6 +30,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
 FP32_MFMA_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix
+BF16_MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix
 
 
 def device_sync():
@@ -46,21 +47,33 @@ def cpu_model():
     return platform.processor()
 
 
-def roofline(phase, info, design):
+def up64(v):
+    return -(-v // 64) * 64
+
+
+def poisson_flops(info):
+    """Multiply-add flops of the four parity-folded transform GEMM launches."""
+    X, Y = up64(info["bins_x"]), up64(info["bins_y"])
+    hx, hy = up64(info["bins_x"] // 2), up64(info["bins_y"] // 2)
+    return (2.0 * X * hy * hy * 2 + 2.0 * hx * (2 * hy) * hx * 2 + 2.0 * (2 * hx) * hy * (2 * hy) * 2
+            + 2.0 * hx * Y * (2 * hx) * 2)
+
+
+def roofline(phase, info, design, poisson_mode=0):
     """Algorithmic work per launch-group of the dominant phase."""
     O, N, P, E = info["num_objects"], info["num_nodes"], info["num_pins"], info["num_nets"]
-    X = -(-info["bins_x"] // 64) * 64
-    Y = -(-info["bins_y"] // 64) * 64
     nb = info["bins_x"] * info["bins_y"]
     sec = phase["seconds"]
     name = phase["name"]
     if name == "poisson":
-        # 4 GEMM launches: rho F_Y^T, F_X T1, phi^ [C_Y|W_Y S_Y], batched Ex/Ey
-        flops = 2.0 * X * Y * Y + 2.0 * X * X * Y + 2.0 * X * Y * (2 * Y) + 2 * (2.0 * X * X * Y)
+        flops = poisson_flops(info)
+        if poisson_mode == 1:  # three bf16 products per FP32 product
+            flops, peak, kern = 3 * flops, BF16_MFMA_PEAK_TFLOPS, "pgemm_kernel<1> x4 (split-BF16)"
+        else:
+            peak, kern = FP32_MFMA_PEAK_TFLOPS, "pgemm_kernel<0> x4 (FP32)"
         ach = flops / sec / 1e12
-        return {"bound": "mfma", "achieved": ach, "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": ach / FP32_MFMA_PEAK_TFLOPS, "traffic": None,
-                "kernel": "gemm_f32_kernel x4", "work_per_launch": flops / 4}
+        return {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
+                "frac": ach / peak, "traffic": None, "kernel": kern, "work_per_launch": flops / 4}
     if name == "wirelength":
         # per pin: obj idx, dx, dy, gathered centre (20 B) + pin gradient (8 B); per net: start + index
         byts = 28.0 * P + 8.0 * E
@@ -108,6 +121,8 @@ def main():
     ap.add_argument("--no-converge", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--poisson-mode", type=int, default=1, choices=[0, 1],
+                    help="transform arithmetic: 0 FP32 MFMA, 1 split-BF16 MFMA")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (scales HBM counters), then exit")
     args = ap.parse_args()
@@ -137,7 +152,7 @@ def main():
     print(f"[bench] design {cells} cells, {nets} nets, {d.num_pins} pins (generated in {t_gen:.1f}s)",
           file=sys.stderr, flush=True)
     t = time.perf_counter()
-    gp = EPlacer2D(d, 1.0)
+    gp = EPlacer2D(d, 1.0, poisson_mode=args.poisson_mode)
     info = gp.info()
     t_create = time.perf_counter() - t
     print(f"[bench] engine ready in {t_create:.1f}s: {info['num_objects']} objects "
@@ -166,7 +181,7 @@ def main():
           f"{r['backtracks']} backtracks", file=sys.stderr, flush=True)
     ph = r["phase_seconds"]
     dominant = max(ph, key=ph.get)
-    roof = roofline({"name": dominant, "seconds": ph[dominant]}, info, d)
+    roof = roofline({"name": dominant, "seconds": ph[dominant]}, info, d, args.poisson_mode)
 
     conv = None
     if not args.no_converge:

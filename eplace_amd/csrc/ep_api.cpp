@@ -152,7 +152,10 @@ int validate(Ctx &c, const ep_design *d, const ep_params *p) {
   // partial-sum slots bound the grid: nx*ny/256 and objects/256 <= kMaxPartials
   if (p->bins_x < 0 || p->bins_y < 0 || p->bins_x > 2048 || p->bins_y > 2048)
     return c.err = "bins out of range [0, 2048]", EP_ERR_ARG;
-  if (p->poisson_mode != 0) return c.err = "poisson_mode must be 0 (FP32 MFMA)", EP_ERR_ARG;
+  if (p->poisson_mode != 0 && p->poisson_mode != 1)
+    return c.err = "poisson_mode must be 0 (FP32 MFMA) or 1 (split-BF16 MFMA)", EP_ERR_ARG;
+  if ((p->bins_x > 0 && p->bins_x % 2) || (p->bins_y > 0 && p->bins_y % 2))
+    return c.err = "bins per side must be even", EP_ERR_ARG;
   if (p->wirelength_model < 0 || p->wirelength_model > 1 || p->exp_mode < 0 || p->exp_mode > 1)
     return c.err = "bad wirelength_model/exp_mode", EP_ERR_ARG;
   return EP_OK;

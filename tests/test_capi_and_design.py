@@ -77,3 +77,12 @@ def test_create_rejects_bad_input_before_any_gpu_work():
         EPlacer2D(d, 1.0, bins=4096)
     with pytest.raises(L.EPError, match="target_density"):
         EPlacer2D(d, 0.0)
+
+
+def test_create_rejects_odd_bins_and_unknown_poisson_mode():
+    from eplace_amd import EPlacer2D
+    d = synthetic(50, 60, seed=1, degrees="c1")
+    with pytest.raises(L.EPError, match="even"):
+        EPlacer2D(d, 1.0, bins=(15, 16))
+    with pytest.raises(L.EPError, match="poisson_mode"):
+        EPlacer2D(d, 1.0, poisson_mode=2)

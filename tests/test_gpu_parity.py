@@ -68,21 +68,53 @@ def test_density_fixed_point_bit_exact(case):
     assert rel(gp.read("rho"), orc.map("rho")) < 1e-6
 
 
-@pytest.mark.parametrize("n", [(16, 16), (64, 64), (128, 64), (256, 256), (512, 512)])
-def test_poisson_vs_fp64(n):
-    say(f"[poisson] grid={n}")
+POISSON_GRIDS = [(16, 16), (10, 6), (64, 64), (128, 64), (96, 160), (256, 256), (512, 512), (1024, 1024)]
+
+
+def poisson_errors(n, mode):
     d = synthetic(200, 200, seed=9, name="p")
-    gp = EPlacer2D(d, 1.0, bins=n)
+    gp = EPlacer2D(d, 1.0, bins=n, poisson_mode=mode)
     i = gp.info()
     rng = np.random.default_rng(7)
     rho = rng.random(n).astype(np.float32) * 2.0
     phi, ex, ey = gp.poisson(rho)
     from oracle.oracle import poisson_f64
     rphi, rex, rey = poisson_f64(rho.astype(np.float64), i["bin_step_x"], i["bin_step_y"], "fast")
-    for a, b, nm in ((ex, rex, "ex"), (ey, rey, "ey"), (phi, rphi, "phi")):
-        e = rel(a, b)
-        say(f"  {nm} rel err {e:.2e}")
+    out = {nm: rel(a, b) for a, b, nm in ((ex, rex, "ex"), (ey, rey, "ey"), (phi, rphi, "phi"))}
+    say(f"  mode {mode} grid {n}: " + ", ".join(f"{k} {v:.2e}" for k, v in out.items()))
+    return out
+
+
+@pytest.mark.parametrize("n", POISSON_GRIDS)
+def test_poisson_vs_fp64(n):
+    """FP32 MFMA transforms: field and potential within 1e-5 of FP64."""
+    say(f"[poisson] grid={n}")
+    for nm, e in poisson_errors(n, 0).items():
         assert e < 1e-5, (nm, e)
+
+
+@pytest.mark.parametrize("n", POISSON_GRIDS)
+def test_poisson_split_bf16_vs_fp64(n):
+    """Split-BF16 (3 products, FP32 accumulate): within 3e-5 of FP64."""
+    say(f"[poisson split-bf16] grid={n}")
+    for nm, e in poisson_errors(n, 1).items():
+        assert e < 3e-5, (nm, e)
+
+
+def test_split_bf16_placement_matches_fp32():
+    """A full run with split-BF16 transforms lands within 0.5% HPWL of FP32."""
+    say("[split-bf16 run] c2_small_64")
+    c = CASES["c2_small_64"]
+    d = synthetic(**c["kw"], name="c2")
+    res = []
+    for mode in (0, 1):
+        gp = EPlacer2D(d, c["td"], bins=c["bins"], poisson_mode=mode)
+        gp.initialization()
+        r = EplaceNesterovOpt(gp).opt()
+        say(f"  mode {mode}: {r['iterations']} iterations, hpwl {r['final_hpwl']:.3f}, tau {r['final_overflow']:.4f}")
+        assert r["final_overflow"] < 0.1
+        res.append(r)
+    assert abs(res[1]["final_hpwl"] - res[0]["final_hpwl"]) <= 0.005 * res[0]["final_hpwl"]
 
 
 @pytest.mark.parametrize("case", list(CASES))
