@@ -9,7 +9,8 @@ A step is one accepted Nesterov iteration of global placement (the plain
 backtracking method of nesterov.hpp:310-369 by default) with every state
 array resident in HBM.  `value` = K timed steps / wall time.  The same run
 also reports the wall time from the first gradient evaluation to overflow
-<= 0.10 and the CPU baseline: the oracle's restatement of the reference path
+<= 0.10 for plain backtracking and for the learned controller
+(eplace_amd/controller_params.npy), and the CPU baseline: the oracle's restatement of the reference path
 (reference float arithmetic, one thread) on a bounded sample of the same
 design.
 
@@ -120,6 +121,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-converge", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-controller convergence run")
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--poisson-mode", type=int, default=1, choices=[0, 1],
                     help="transform arithmetic: 0 FP32 MFMA, 1 split-BF16 MFMA")
@@ -199,6 +201,24 @@ def main():
         print(f"[bench] converged: {rc['iterations']} iterations in {tc:.3f}s, overflow {rc['final_overflow']:.4f}, "
               f"HPWL {rc['final_hpwl']:.6g}", file=sys.stderr, flush=True)
         del restart_t0
+        if args.method == "bktrk" and not args.no_learned:
+            from eplace_amd.controller import Controller
+            ctrl = Controller.load()
+            gp.write("positions", init_pos)
+            device_sync()
+            t0 = time.perf_counter()
+            gp.initialization()
+            rl = EplaceNesterovOpt(gp, method="learned", controller=ctrl, target_overflow=0.10).opt()
+            device_sync()
+            tl = time.perf_counter() - t0
+            conv["learned"] = {"seconds_to_overflow_0.10": tl, "iterations": rl["iterations"],
+                               "evaluations": rl["evaluations"], "backtracks": rl["backtracks"],
+                               "controller_fallbacks": rl["controller_fallbacks"],
+                               "final_overflow": rl["final_overflow"], "final_hpwl": rl["final_hpwl"],
+                               "speedup_vs_plain": tc / tl, "hpwl_gap_vs_plain": rl["final_hpwl"] / rc["final_hpwl"] - 1}
+            print(f"[bench] learned: {rl['iterations']} iterations in {tl:.3f}s ({tc / tl:.2f}x), "
+                  f"overflow {rl['final_overflow']:.4f}, HPWL {rl['final_hpwl']:.6g} "
+                  f"({100 * (rl['final_hpwl'] / rc['final_hpwl'] - 1):+.3f}%)", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:

@@ -16,6 +16,14 @@ EP_OK = 0
 EP_ERR_ARG, EP_ERR_HIP, EP_ERR_STATE, EP_ERR_NUMERIC, EP_ERR_NOMEM = 1, 2, 3, 4, 5
 EP_STAGE_MGP, EP_STAGE_FILLERONLY, EP_STAGE_CGP = 0, 1, 2
 
+EP_LOG_COLS = 24
+EP_LOG_ITER, EP_LOG_TAU, EP_LOG_LAMBDA, EP_LOG_HPWL, EP_LOG_STEP, EP_LOG_INV_GAMMA, EP_LOG_EVALS, \
+    EP_LOG_BACKTRACKS, EP_LOG_FEATURES = range(9)
+EP_LOG_STEP_PROP, EP_LOG_STEP_USED, EP_LOG_LAMBDA_BEFORE, EP_LOG_FALLBACKS = 20, 21, 22, 23
+EP_CTRL_FEATURES, EP_CTRL_HIDDEN, EP_CTRL_OUTPUTS = 12, 64, 2
+EP_CTRL_WEIGHTS = 12 * 64 + 64 + 64 * 64 + 64 + 2 * 64 + 2
+EP_CTRL_PARAMS = EP_CTRL_WEIGHTS + 2 * 12 + 2 * 2
+
 (EP_ARR_POSITIONS, EP_ARR_NODE_Q, EP_ARR_FILLER_Q, EP_ARR_RHO, EP_ARR_PHI, EP_ARR_EX, EP_ARR_EY,
  EP_ARR_GRAD_DENSITY, EP_ARR_GRAD_WL, EP_ARR_GRAD_TOTAL, EP_ARR_TERM_MAP, EP_ARR_BASE_MAP,
  EP_ARR_PIN_GRAD, EP_ARR_MAIN) = range(14)
@@ -68,7 +76,7 @@ class EPOptParams(C.Structure):
     _fields_ = [
         ("method", C.c_int32), ("target_overflow", C.c_float), ("max_iterations", C.c_int32),
         ("max_backtracks", C.c_int32), ("sync_every", C.c_int32), ("log_capacity", C.c_int32),
-        ("log", C.c_void_p), ("phase_seconds", C.c_void_p),
+        ("log", C.c_void_p), ("phase_seconds", C.c_void_p), ("controller", C.c_void_p),
     ]
 
 
@@ -76,7 +84,7 @@ class EPRunStats(C.Structure):
     _fields_ = [
         ("iterations", C.c_int32), ("evaluations", C.c_int32), ("backtracks", C.c_int32),
         ("stopped", C.c_int32), ("final_overflow", C.c_double), ("final_hpwl", C.c_double),
-        ("seconds", C.c_double),
+        ("seconds", C.c_double), ("controller_fallbacks", C.c_int32), ("pad", C.c_int32),
     ]
 
 
@@ -101,6 +109,14 @@ SYMBOLS = {
     "ep_read": ([C.c_void_p, C.c_int32, C.c_void_p, C.c_int64], C.c_int),
     "ep_write": ([C.c_void_p, C.c_int32, C.c_void_p, C.c_int64], C.c_int),
     "ep_last_timings": ([C.c_void_p, C.c_void_p], C.c_int),
+    "ep_ctrl_create": ([C.c_int32, C.c_void_p, C.c_uint32, C.c_void_p], C.c_int),
+    "ep_ctrl_destroy": ([C.c_void_p], C.c_int),
+    "ep_ctrl_last_error": ([C.c_void_p], C.c_char_p),
+    "ep_ctrl_get_params": ([C.c_void_p, C.c_void_p], C.c_int),
+    "ep_ctrl_set_params": ([C.c_void_p, C.c_void_p], C.c_int),
+    "ep_ctrl_train": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_float, C.c_void_p], C.c_int),
+    "ep_ctrl_predict": ([C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p], C.c_int),
+    "ep_ctrl_loss_grad": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p], C.c_int),
 }
 
 _lib = None

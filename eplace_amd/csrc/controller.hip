@@ -1,0 +1,454 @@
+// Learned step / lambda controller: a 12 -> 64 -> 64 -> 2 MLP (tanh hidden
+// layers, FP32) with hand-written inference, backward and Adam kernels.
+// No counterpart in the reference; it drives method 2 of the optimiser (the
+// reference's backtracking Nesterov, nesterov.hpp:310-369, with the first
+// trial step proposed by the network and the penalty multiplier of
+// updatePenaltyFactor, eplace.cpp:846-883, scaled by the network's factor).
+//
+// Features of iteration k, computed on the device at y_k (after the
+// combine at the current lambda and the plain step proposal 1/L):
+//   0 tau_k                      1 tau_k - tau_{k-1}
+//   2 log(lambda_k / lambda_0)   3 log(HPWL_k / HPWL_0)
+//   4 100 (HPWL_k - HPWL_{k-1}) / HPWL_k
+//   5 log(|grad WL| / (lambda_k |grad D|))
+//   6 log(s_{k-1} / s_0)         7 log(s_{k-1} / s_{k-2})   8 log(s_{k-2} / s_{k-3})
+//   9 k / 1000 (MAX_ITERATION)   10 log(proposal_k / s_{k-1})
+//   11 log(lambda_k / lambda_{k-1})
+// with s_j the accepted 1/L' of iteration j; terms that need missing history
+// are 0.  Every feature is free of the design's absolute scale (the
+// reference's penalty rule, which is not, is applied by the device itself).
+// Outputs: o0 = log(step / proposal), clamped to [log 1/4, 0] (the
+// controller only shortens the plain proposal), and
+// o1 = log of the factor applied on top of the reference's penalty
+// multiplier, clamped to [-0.05, 0.1]; the product stays inside the
+// reference's multiplier range [0.95, 1.05].  o1 = 0 reproduces the
+// reference.
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "ep_internal.h"
+#include "reduce.h"
+
+namespace ep {
+
+constexpr int F = EP_CTRL_FEATURES, H = EP_CTRL_HIDDEN, NO = EP_CTRL_OUTPUTS;
+constexpr int oW1 = 0, ob1 = oW1 + H * F, oW2 = ob1 + H, ob2 = oW2 + H * H, oW3 = ob2 + H, ob3 = oW3 + NO * H;
+constexpr int NW = ob3 + NO;  // trained weights
+constexpr int oXmu = NW, oXsd = oXmu + F, oYmu = oXsd + F, oYsd = oYmu + NO;
+constexpr int NP = oYsd + NO;
+static_assert(NW == EP_CTRL_WEIGHTS && NP == EP_CTRL_PARAMS, "controller parameter layout");
+constexpr int kSamplesPerBlock = 32;
+
+struct CtrlDev {
+  std::string err;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  float *P = nullptr, *m = nullptr, *v = nullptr;  // parameters, Adam moments
+  int adam_t = 0;
+};
+
+const float *ctrl_params(const CtrlDev *c) { return c->P; }
+
+// One sample through the network with the weights staged in LDS: x (F
+// normalised inputs) -> h1, h2 -> out (normalised outputs).  64 threads.
+__device__ __forceinline__ void mlp_forward(const float *W, const float *x, float *h1, float *h2, float *out) {
+  const int t = threadIdx.x;
+  float a = W[ob1 + t];
+#pragma unroll
+  for (int i = 0; i < F; ++i) a += W[oW1 + t * F + i] * x[i];
+  h1[t] = tanhf(a);
+  __syncthreads();
+  float b = W[ob2 + t];
+  for (int j = 0; j < H; ++j) b += W[oW2 + t * H + j] * h1[j];
+  h2[t] = tanhf(b);
+  __syncthreads();
+  if (t < NO) {
+    float o = W[ob3 + t];
+    for (int j = 0; j < H; ++j) o += W[oW3 + t * H + j] * h2[j];
+    out[t] = o;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void stage_weights(const float *__restrict__ P, float *W) {
+  for (int i = threadIdx.x; i < NP; i += blockDim.x) W[i] = P[i];
+  __syncthreads();
+}
+
+__device__ __forceinline__ float flog(double v) { return isfinite(v) ? (float)v : 0.f; }
+
+// Features of iteration k and, when active, the controller's step and lambda
+// multiplier.  One block of H threads.
+__global__ void __launch_bounds__(H) ctrl_step_kernel(DevScalars *sc, const float *__restrict__ P, int active) {
+  if (sc->done) return;
+  __shared__ float W[NP];
+  __shared__ float x[F], h1[H], h2[H], out[NO];
+  if (threadIdx.x == 0) {
+    const int k = sc->iter;
+    const double tau = sc->overflow, lam = sc->lambda, hp = sc->hpwl, prop = sc->step;
+    float f[F];
+    f[0] = (float)tau;
+    f[1] = k > 0 ? (float)(tau - sc->tau_prev) : 0.f;
+    f[2] = flog(log(lam / sc->lambda0));
+    f[3] = flog(log(hp / sc->hpwl0));
+    f[4] = k > 0 ? flog(100.0 * (hp - sc->hpwl_prev) / hp) : 0.f;
+    f[5] = flog(log(sqrt(sc->gwl_norm2) / (lam * sqrt(sc->gd_norm2))));
+    f[6] = k >= 1 ? flog(log(sc->step_hist[0] / sc->step0)) : 0.f;
+    f[7] = k >= 2 ? flog(log(sc->step_hist[0] / sc->step_hist[1])) : 0.f;
+    f[8] = k >= 3 ? flog(log(sc->step_hist[1] / sc->step_hist[2])) : 0.f;
+    f[9] = (float)k / 1000.f;
+    f[10] = k >= 1 ? flog(log(prop / sc->step_hist[0])) : 0.f;
+    f[11] = k > 0 ? flog(log(lam / sc->lambda_before)) : 0.f;
+    for (int i = 0; i < F; ++i) {
+      sc->feat[i] = f[i];
+      x[i] = f[i];
+    }
+    sc->step_prop = prop;
+    sc->lambda_before = lam;
+    sc->tau_prev = tau;
+    sc->hpwl_prev = hp;
+    sc->backtracks_mark = sc->backtracks;
+    sc->trial = 0;
+    sc->ctrl_active = active;
+    sc->lam_mult = 1.0;
+    sc->step_used = prop;
+  }
+  __syncthreads();
+  if (!active) return;
+  stage_weights(P, W);
+  if (threadIdx.x < F) x[threadIdx.x] = (x[threadIdx.x] - W[oXmu + threadIdx.x]) / W[oXsd + threadIdx.x];
+  __syncthreads();
+  mlp_forward(W, x, h1, h2, out);
+  if (threadIdx.x == 0) {
+    const float o0 = out[0] * W[oYsd] + W[oYmu], o1 = out[1] * W[oYsd + 1] + W[oYmu + 1];
+    const double s = fmin(fmax((double)o0, -1.3862943611198906), 0.0);
+    const double l = fmin(fmax((double)o1, -0.05), 0.1);
+    if (isfinite(s) && isfinite(l)) {
+      sc->step = sc->step_prop * exp(s);
+      sc->lam_mult = exp(l);
+    }
+    sc->step_used = sc->step;
+  }
+}
+
+void launch_ctrl_step(Ctx &c, const CtrlDev *ctrl, int active) {
+  hipLaunchKernelGGL(ctrl_step_kernel, dim3(1), dim3(H), 0, c.stream, c.sc, ctrl ? ctrl->P : nullptr,
+                     ctrl ? active : 0);
+}
+
+// ---------------------------------------------------------------------------
+// Training: mean over samples of sum over outputs of (out - (y - mu)/sd)^2.
+// Each block takes kSamplesPerBlock samples, thread t owns hidden unit t
+// (row t of W1 and W2, column t of W3) and accumulates its gradient in
+// registers; per-block partial gradients are summed in a fixed order.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(H) mlp_grad_kernel(const float *__restrict__ P, const float *__restrict__ X,
+                                                    const float *__restrict__ Y, int n, float *gpart,
+                                                    float *lpart) {
+  __shared__ float W[NP];
+  __shared__ float x[F], h1[H], h2[H], out[NO], dout[NO], dz2[H];
+  stage_weights(P, W);
+  const int t = threadIdx.x;
+  float gw1[F], gw2[H], gb1 = 0.f, gb2 = 0.f, gw3[NO] = {0.f, 0.f}, gb3 = 0.f, loss = 0.f;
+#pragma unroll
+  for (int i = 0; i < F; ++i) gw1[i] = 0.f;
+#pragma unroll
+  for (int j = 0; j < H; ++j) gw2[j] = 0.f;
+  const int s0 = blockIdx.x * kSamplesPerBlock, s1 = min(n, s0 + kSamplesPerBlock);
+  const float inv_n = 1.f / (float)n;
+  for (int s = s0; s < s1; ++s) {
+    if (t < F) x[t] = (X[(size_t)s * F + t] - W[oXmu + t]) / W[oXsd + t];
+    __syncthreads();
+    mlp_forward(W, x, h1, h2, out);
+    if (t < NO) {
+      const float d = out[t] - (Y[(size_t)s * NO + t] - W[oYmu + t]) / W[oYsd + t];
+      dout[t] = 2.f * d * inv_n;
+      loss += d * d;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int o = 0; o < NO; ++o) gw3[o] += dout[o] * h2[t];
+    if (t < NO) gb3 += dout[t];
+    float dh2 = 0.f;
+#pragma unroll
+    for (int o = 0; o < NO; ++o) dh2 += dout[o] * W[oW3 + o * H + t];
+    const float z2 = dh2 * (1.f - h2[t] * h2[t]);
+    dz2[t] = z2;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < H; ++j) gw2[j] += z2 * h1[j];
+    gb2 += z2;
+    float dh1 = 0.f;
+    for (int j = 0; j < H; ++j) dh1 += dz2[j] * W[oW2 + j * H + t];
+    const float z1 = dh1 * (1.f - h1[t] * h1[t]);
+#pragma unroll
+    for (int i = 0; i < F; ++i) gw1[i] += z1 * x[i];
+    gb1 += z1;
+    __syncthreads();
+  }
+  float *g = gpart + (size_t)blockIdx.x * NW;
+#pragma unroll
+  for (int i = 0; i < F; ++i) g[oW1 + t * F + i] = gw1[i];
+  g[ob1 + t] = gb1;
+#pragma unroll
+  for (int j = 0; j < H; ++j) g[oW2 + t * H + j] = gw2[j];
+  g[ob2 + t] = gb2;
+#pragma unroll
+  for (int o = 0; o < NO; ++o) g[oW3 + o * H + t] = gw3[o];
+  if (t < NO) g[ob3 + t] = gb3;
+  // block loss: lanes 0..NO-1 hold their output's sum
+  float l = loss;
+  for (int o = 1; o < NO; ++o) l += __shfl(loss, o, 64);
+  if (t == 0) lpart[blockIdx.x] = l;
+}
+
+// g = sum over blocks of the partial gradients (fixed order); update = 1
+// applies one Adam step to the weights, else g is stored in gout.
+__global__ void adam_kernel(float *P, float *m, float *v, const float *__restrict__ gpart, int nblocks, float lr,
+                            float bc1, float bc2, int update, float *gout) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= NW) return;
+  float g = 0.f;
+  for (int b = 0; b < nblocks; ++b) g += gpart[(size_t)b * NW + p];
+  if (!update) {
+    gout[p] = g;
+    return;
+  }
+  const float mm = 0.9f * m[p] + 0.1f * g;
+  const float vv = 0.999f * v[p] + 0.001f * g * g;
+  m[p] = mm;
+  v[p] = vv;
+  P[p] -= lr * (mm / bc1) / (sqrtf(vv / bc2) + 1e-8f);
+}
+
+__global__ void loss_sum_kernel(const float *__restrict__ lpart, int nblocks, int n, float *dst) {
+  double s = 0;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += lpart[b];
+  s = block_sum(s);
+  if (threadIdx.x == 0) *dst = (float)(s / n);
+}
+
+__global__ void __launch_bounds__(H) mlp_predict_kernel(const float *__restrict__ P, const float *__restrict__ X,
+                                                       int n, float *Y) {
+  __shared__ float W[NP];
+  __shared__ float x[F], h1[H], h2[H], out[NO];
+  stage_weights(P, W);
+  const int t = threadIdx.x;
+  const int s0 = blockIdx.x * kSamplesPerBlock, s1 = min(n, s0 + kSamplesPerBlock);
+  for (int s = s0; s < s1; ++s) {
+    if (t < F) x[t] = (X[(size_t)s * F + t] - W[oXmu + t]) / W[oXsd + t];
+    __syncthreads();
+    mlp_forward(W, x, h1, h2, out);
+    if (t < NO) Y[(size_t)s * NO + t] = out[t] * W[oYsd + t] + W[oYmu + t];
+    __syncthreads();
+  }
+}
+
+}  // namespace ep
+
+struct ep_ctrl {
+  ep::CtrlDev d;
+};
+
+namespace ep {
+const CtrlDev *ctrl_of(const ep_ctrl *x) { return x ? &x->d : nullptr; }
+int ctrl_device(const CtrlDev *c) { return c->device; }
+}  // namespace ep
+
+namespace {
+
+using ep::CtrlDev;
+
+#define CT_HIP(call)                                                           \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess) {                                                    \
+      c.err = std::string(#call) + ": " + hipGetErrorString(e_);              \
+      return EP_ERR_HIP;                                                       \
+    }                                                                          \
+  } while (0)
+
+void init_params(std::vector<float> &p, uint32_t seed) {
+  std::mt19937 rng(seed ? seed : 1);
+  auto fill = [&](int off, int rows, int cols) {
+    std::normal_distribution<float> nd(0.f, std::sqrt(1.f / (float)cols));
+    for (int i = 0; i < rows * cols; ++i) p[off + i] = nd(rng);
+  };
+  std::fill(p.begin(), p.end(), 0.f);
+  fill(ep::oW1, ep::H, ep::F);
+  fill(ep::oW2, ep::H, ep::H);
+  fill(ep::oW3, ep::NO, ep::H);
+  for (int i = 0; i < ep::F; ++i) p[ep::oXsd + i] = 1.f;
+  for (int o = 0; o < ep::NO; ++o) p[ep::oYsd + o] = 1.f;
+}
+
+int check_params(CtrlDev &c, const float *p) {
+  for (int i = 0; i < ep::NP; ++i)
+    if (!std::isfinite(p[i])) return c.err = "non-finite controller parameter", EP_ERR_ARG;
+  for (int i = 0; i < ep::F; ++i)
+    if (!(p[ep::oXsd + i] > 0.f)) return c.err = "input std must be > 0", EP_ERR_ARG;
+  for (int o = 0; o < ep::NO; ++o)
+    if (!(p[ep::oYsd + o] > 0.f)) return c.err = "output std must be > 0", EP_ERR_ARG;
+  return EP_OK;
+}
+
+int upload_params(CtrlDev &c, const float *p) {
+  int rc = check_params(c, p);
+  if (rc) return rc;
+  CT_HIP(hipMemcpy(c.P, p, ep::NP * sizeof(float), hipMemcpyHostToDevice));
+  CT_HIP(hipMemset(c.m, 0, ep::NW * sizeof(float)));
+  CT_HIP(hipMemset(c.v, 0, ep::NW * sizeof(float)));
+  c.adam_t = 0;
+  return EP_OK;
+}
+
+struct Batch {
+  float *X = nullptr, *Y = nullptr, *gpart = nullptr, *lpart = nullptr, *scalar = nullptr;
+  int nblocks = 0;
+  ~Batch() {
+    void *p[] = {X, Y, gpart, lpart, scalar};
+    for (void *q : p)
+      if (q) (void)hipFree(q);
+  }
+};
+
+int make_batch(CtrlDev &c, Batch &b, const float *X, const float *Y, int64_t n) {
+  if (n <= 0 || n > (1 << 24)) return c.err = "sample count out of range [1, 2^24]", EP_ERR_ARG;
+  b.nblocks = (int)((n + ep::kSamplesPerBlock - 1) / ep::kSamplesPerBlock);
+  CT_HIP(hipMalloc(&b.X, (size_t)n * ep::F * sizeof(float)));
+  CT_HIP(hipMemcpy(b.X, X, (size_t)n * ep::F * sizeof(float), hipMemcpyHostToDevice));
+  if (Y) {
+    CT_HIP(hipMalloc(&b.Y, (size_t)n * ep::NO * sizeof(float)));
+    CT_HIP(hipMemcpy(b.Y, Y, (size_t)n * ep::NO * sizeof(float), hipMemcpyHostToDevice));
+    CT_HIP(hipMalloc(&b.gpart, (size_t)b.nblocks * ep::NW * sizeof(float)));
+    CT_HIP(hipMalloc(&b.lpart, (size_t)b.nblocks * sizeof(float)));
+  }
+  CT_HIP(hipMalloc(&b.scalar, ep::NW * sizeof(float)));
+  return EP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ep_ctrl_create(int32_t device, const float *params, uint32_t seed, ep_ctrl **out) {
+  if (!out) return EP_ERR_ARG;
+  *out = nullptr;
+  ep_ctrl *x = new ep_ctrl();
+  *out = x;
+  CtrlDev &c = x->d;
+  c.device = device;
+  CT_HIP(hipSetDevice(device));
+  CT_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  CT_HIP(hipMalloc(&c.P, ep::NP * sizeof(float)));
+  CT_HIP(hipMalloc(&c.m, ep::NW * sizeof(float)));
+  CT_HIP(hipMalloc(&c.v, ep::NW * sizeof(float)));
+  std::vector<float> p(ep::NP);
+  if (params) std::memcpy(p.data(), params, ep::NP * sizeof(float));
+  else init_params(p, seed);
+  return upload_params(c, p.data());
+}
+
+int ep_ctrl_destroy(ep_ctrl *x) {
+  if (!x) return EP_ERR_ARG;
+  CtrlDev &c = x->d;
+  (void)hipSetDevice(c.device);
+  if (c.stream) (void)hipStreamSynchronize(c.stream);
+  void *p[] = {c.P, c.m, c.v};
+  for (void *q : p)
+    if (q) (void)hipFree(q);
+  if (c.stream) (void)hipStreamDestroy(c.stream);
+  delete x;
+  return EP_OK;
+}
+
+const char *ep_ctrl_last_error(const ep_ctrl *x) { return x ? x->d.err.c_str() : "null controller"; }
+
+int ep_ctrl_get_params(ep_ctrl *x, float *params) {
+  if (!x || !params) return EP_ERR_ARG;
+  CtrlDev &c = x->d;
+  CT_HIP(hipSetDevice(c.device));
+  CT_HIP(hipStreamSynchronize(c.stream));
+  CT_HIP(hipMemcpy(params, c.P, ep::NP * sizeof(float), hipMemcpyDeviceToHost));
+  return EP_OK;
+}
+
+int ep_ctrl_set_params(ep_ctrl *x, const float *params) {
+  if (!x || !params) return EP_ERR_ARG;
+  CtrlDev &c = x->d;
+  CT_HIP(hipSetDevice(c.device));
+  CT_HIP(hipStreamSynchronize(c.stream));
+  return upload_params(c, params);
+}
+
+int ep_ctrl_train(ep_ctrl *x, const float *X, const float *Y, int64_t n, int32_t steps, float lr,
+                  float *loss_out) {
+  if (!x || !X || !Y || steps < 0 || !(lr > 0.f)) return EP_ERR_ARG;
+  CtrlDev &c = x->d;
+  CT_HIP(hipSetDevice(c.device));
+  Batch b;
+  int rc = make_batch(c, b, X, Y, n);
+  if (rc) return rc;
+  float *dloss = nullptr;
+  if (loss_out && steps > 0) CT_HIP(hipMalloc(&dloss, (size_t)steps * sizeof(float)));
+  for (int s = 0; s < steps; ++s) {
+    hipLaunchKernelGGL(ep::mlp_grad_kernel, dim3(b.nblocks), dim3(ep::H), 0, c.stream, c.P, b.X, b.Y, (int)n,
+                       b.gpart, b.lpart);
+    if (dloss)
+      hipLaunchKernelGGL(ep::loss_sum_kernel, dim3(1), dim3(256), 0, c.stream, b.lpart, b.nblocks, (int)n,
+                         dloss + s);
+    ++c.adam_t;
+    const float bc1 = 1.f - std::pow(0.9f, (float)c.adam_t), bc2 = 1.f - std::pow(0.999f, (float)c.adam_t);
+    hipLaunchKernelGGL(ep::adam_kernel, dim3((ep::NW + 255) / 256), dim3(256), 0, c.stream, c.P, c.m, c.v, b.gpart,
+                       b.nblocks, lr, bc1, bc2, 1, nullptr);
+  }
+  hipError_t e = hipStreamSynchronize(c.stream);
+  if (e == hipSuccess && dloss) e = hipMemcpy(loss_out, dloss, (size_t)steps * sizeof(float), hipMemcpyDeviceToHost);
+  if (dloss) (void)hipFree(dloss);
+  if (e != hipSuccess) return c.err = std::string("training: ") + hipGetErrorString(e), EP_ERR_HIP;
+  return EP_OK;
+}
+
+int ep_ctrl_predict(ep_ctrl *x, const float *X, int64_t n, float *Y) {
+  if (!x || !X || !Y) return EP_ERR_ARG;
+  CtrlDev &c = x->d;
+  CT_HIP(hipSetDevice(c.device));
+  Batch b;
+  int rc = make_batch(c, b, X, nullptr, n);
+  if (rc) return rc;
+  float *dy = nullptr;
+  CT_HIP(hipMalloc(&dy, (size_t)n * ep::NO * sizeof(float)));
+  hipLaunchKernelGGL(ep::mlp_predict_kernel, dim3(b.nblocks), dim3(ep::H), 0, c.stream, c.P, b.X, (int)n, dy);
+  hipError_t e = hipStreamSynchronize(c.stream);
+  if (e == hipSuccess) e = hipMemcpy(Y, dy, (size_t)n * ep::NO * sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipFree(dy);
+  if (e != hipSuccess) return c.err = std::string("predict: ") + hipGetErrorString(e), EP_ERR_HIP;
+  return EP_OK;
+}
+
+int ep_ctrl_loss_grad(ep_ctrl *x, const float *X, const float *Y, int64_t n, float *grad, float *loss) {
+  if (!x || !X || !Y || !grad) return EP_ERR_ARG;
+  CtrlDev &c = x->d;
+  CT_HIP(hipSetDevice(c.device));
+  Batch b;
+  int rc = make_batch(c, b, X, Y, n);
+  if (rc) return rc;
+  hipLaunchKernelGGL(ep::mlp_grad_kernel, dim3(b.nblocks), dim3(ep::H), 0, c.stream, c.P, b.X, b.Y, (int)n,
+                     b.gpart, b.lpart);
+  hipLaunchKernelGGL(ep::adam_kernel, dim3((ep::NW + 255) / 256), dim3(256), 0, c.stream, c.P, c.m, c.v, b.gpart,
+                     b.nblocks, 0.f, 1.f, 1.f, 0, b.scalar);
+  float *dl = nullptr;
+  CT_HIP(hipMalloc(&dl, sizeof(float)));
+  hipLaunchKernelGGL(ep::loss_sum_kernel, dim3(1), dim3(256), 0, c.stream, b.lpart, b.nblocks, (int)n, dl);
+  hipError_t e = hipStreamSynchronize(c.stream);
+  if (e == hipSuccess) e = hipMemcpy(grad, b.scalar, ep::NW * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && loss) e = hipMemcpy(loss, dl, sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipFree(dl);
+  if (e != hipSuccess) return c.err = std::string("loss_grad: ") + hipGetErrorString(e), EP_ERR_HIP;
+  return EP_OK;
+}
+
+}  // extern "C"

@@ -315,7 +315,8 @@ __global__ void __launch_bounds__(kBlock) gather_big_kernel(const int32_t *__res
 // node's pins in modulePins order (eplace.cpp:531-549), preconditioned
 // combine (eplace.cpp:694-737) and, optionally, the Lipschitz partial sums
 // |g - g_prev|^2 and |y - y_prev|^2 over the stage's objects
-// (optutils.hpp:19-35).  Blocks walk the objects XCD-contiguously: with
+// (optutils.hpp:19-35) with, for the step controller, |grad WL|^2 and
+// |grad D|^2 over the stage.  Blocks walk the objects XCD-contiguously: with
 // nets stored by their lowest node, the pin gradients a range of nodes
 // reads lie in a narrow band of the pin array.
 __global__ void __launch_bounds__(kBlock) gather_combine_kernel(
@@ -329,7 +330,7 @@ __global__ void __launch_bounds__(kBlock) gather_combine_kernel(
     const float2 *__restrict__ grad_prev, double *partials, const DevScalars *sc) {
   if (sc->done) return;
   int o = xcd_block() * blockDim.x + threadIdx.x;
-  double ln = 0, ld = 0;
+  double ln = 0, ld = 0, nw = 0, nd = 0;
   if (o < O) {
     uint8_t k = kind[o];
     float2 d;
@@ -381,14 +382,20 @@ __global__ void __launch_bounds__(kBlock) gather_combine_kernel(
       double dxx = (double)rn.x - rp.x, dxy = (double)rn.y - rp.y;
       ln = dgx * dgx + dgy * dgy;
       ld = dxx * dxx + dxy * dxy;
+      nw = (double)wl.x * wl.x + (double)wl.y * wl.y;
+      nd = (double)d.x * d.x + (double)d.y * d.y;
     }
   }
   if (lipschitz) {
     ln = block_sum(ln);
     ld = block_sum(ld);
+    nw = block_sum(nw);
+    nd = block_sum(nd);
     if (threadIdx.x == 0) {
       partials[kPartialLip + 2 * blockIdx.x] = ln;
       partials[kPartialLip + 2 * blockIdx.x + 1] = ld;
+      partials[kPartialNorm + 2 * blockIdx.x] = nw;
+      partials[kPartialNorm + 2 * blockIdx.x + 1] = nd;
     }
   }
 }

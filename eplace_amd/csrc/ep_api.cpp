@@ -578,7 +578,9 @@ int set_stage_positions(Ctx &c, const float *xy, int64_t n) {
   return rc;
 }
 
-// Device-resident EplaceNesterovOpt::opt (nesterov.hpp:207-223 + opt.hpp:29-36).
+// Device-resident EplaceNesterovOpt::opt (nesterov.hpp:207-223 + opt.hpp:29-36);
+// method 2 is the backtracking method with the controller's first trial step
+// and lambda multiplier.
 int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
   ep_opt_params o = {};
   o.method = 0;
@@ -590,11 +592,15 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
   if (o.sync_every <= 0) o.sync_every = 16;
   if (o.max_backtracks <= 0) o.max_backtracks = 30;
   if (o.log_capacity < 0 || (o.log_capacity > 0 && !o.log)) return c.err = "bad log buffer", EP_ERR_ARG;
+  if (o.method < 0 || o.method > 2) return c.err = "method must be 0 (vanilla), 1 (backtracking) or 2 (learned)", EP_ERR_ARG;
+  const CtrlDev *ctrl = ctrl_of(o.controller);
+  if (o.method == 2 && !ctrl) return c.err = "method 2 needs a controller", EP_ERR_ARG;
+  if (ctrl && ctrl_device(ctrl) != c.device) return c.err = "controller is on another device", EP_ERR_ARG;
   double *dlog = nullptr;
   int cap = o.log_capacity;
   if (cap > 0) {
-    EP_HIP(hipMalloc(&dlog, (size_t)cap * 8 * sizeof(double)));
-    EP_HIP(hipMemset(dlog, 0, (size_t)cap * 8 * sizeof(double)));
+    EP_HIP(hipMalloc(&dlog, (size_t)cap * EP_LOG_COLS * sizeof(double)));
+    EP_HIP(hipMemset(dlog, 0, (size_t)cap * EP_LOG_COLS * sizeof(double)));
   }
   if (o.phase_seconds) {
     // events for every evaluation, created before the timed loop
@@ -621,13 +627,24 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
     h.stage = c.stage;
     h.max_iter = o.max_iterations;
     h.target_overflow = o.target_overflow;
+    h.lambda0 = h.lambda;
+    h.hpwl0 = h.hpwl;
+    h.step0 = 1;
+    h.step_hist[0] = h.step_hist[1] = h.step_hist[2] = 1;
+    h.tau_prev = h.overflow;
+    h.hpwl_prev = h.hpwl;
+    h.step_prop = h.step_used = 1;
+    h.lambda_before = h.lambda;
+    h.lam_mult = 1;
+    h.backtracks_mark = h.trial = h.fallbacks = h.ctrl_active = 0;
   });
   if (rc) return rc;
   launch_positions_to_ref(c, c.cur_main);
   // stop check before the first step uses tau of the last evaluation
   launch_done_check_ex(c, 0, nullptr, 0, 0);
   int k = 0;
-  bool bk = o.method == 1;
+  const bool bk = o.method >= 1, learned = o.method == 2;
+  const bool features = learned || dlog;  // controller inputs are logged for training
   while (true) {
     if (!bk) {
       int chunk = o.sync_every;
@@ -646,6 +663,7 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
       if (k == 0 && !c.eval_valid) evaluate(c, false, true, nullptr, c.last_ref, c.last_grad, false);
       else launch_gather_combine(c, false, true, nullptr, c.last_ref, c.last_grad);
       launch_step_scalar(c, 1);
+      if (features) launch_ctrl_step(c, ctrl, learned ? 1 : 0);
       launch_positions_to_ref(c, c.cur_ref);
       launch_copy_float2(c, c.cur_grad, c.gtot, c.O);
       int tries = 0;
@@ -661,8 +679,9 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
       std::swap(c.last_ref, c.cur_ref);
       std::swap(c.last_grad, c.cur_grad);
       std::swap(c.cur_main, c.new_main);
-      // updatePenaltyFactor with the HPWL of the accepted evaluation
-      launch_reduce_wirelength(c, 1);
+      // updatePenaltyFactor with the HPWL of the accepted evaluation, or the
+      // controller's multiplier
+      launch_reduce_wirelength(c, learned ? 2 : 1);
       launch_done_check_ex(c, k + 1, dlog, cap, 1);
       ++k;
     }
@@ -711,7 +730,7 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
     o.phase_seconds[5] = (double)n;
   }
   if (cap > 0) {
-    EP_HIP(hipMemcpy(o.log, dlog, (size_t)cap * 8 * sizeof(double), hipMemcpyDeviceToHost));
+    EP_HIP(hipMemcpy(o.log, dlog, (size_t)cap * EP_LOG_COLS * sizeof(double), hipMemcpyDeviceToHost));
     (void)hipFree(dlog);
   }
   if (out) {
@@ -721,6 +740,7 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
     out->stopped = 1;
     out->final_overflow = final_tau;
     out->seconds = std::chrono::duration<double>(t1 - t0).count();
+    out->controller_fallbacks = h.fallbacks;
   }
   return EP_OK;
 }

@@ -60,6 +60,14 @@ struct DevScalars {
   int iter, done, accepted, stage;
   int evals, backtracks, max_iter, bad_numeric;
   float target_overflow, pad0;
+  // controller features and actions (backtracking / learned runs)
+  double gwl_norm2, gd_norm2;         // sum |grad WL|^2, sum |grad D|^2 of the last combine
+  double lambda0, hpwl0, step0;       // at the start of the run / first accepted step
+  double step_hist[3];                // accepted 1/L' of the last three iterations
+  double tau_prev, hpwl_prev;         // at y_{k-1}
+  double step_prop, step_used, lambda_before, lam_mult;
+  float feat[EP_CTRL_FEATURES];
+  int backtracks_mark, trial, fallbacks, ctrl_active;
 };
 
 // Slots of the partial-sum buffer (doubles).
@@ -69,7 +77,8 @@ constexpr int kPartialWL = kMaxPartials;        // hpwl, wl pairs
 constexpr int kPartialLip = 3 * kMaxPartials;   // num, den pairs
 constexpr int kPartialLambda = 5 * kMaxPartials;  // |gwl|, |gd| pairs
 constexpr int kPartialRho = 7 * kMaxPartials;      // sum of rho per block
-constexpr int kPartialTotal = 8 * kMaxPartials;
+constexpr int kPartialNorm = 8 * kMaxPartials;     // |gwl|^2, |gd|^2 pairs
+constexpr int kPartialTotal = 10 * kMaxPartials;
 
 struct Poisson;  // poisson.hip
 
@@ -209,6 +218,12 @@ void launch_nesterov_move_ex(Ctx &c, const float2 *ref, const float2 *grad, cons
                              float2 *main_out, float2 *ref_unclamped_out, float2 *save_ref, float2 *save_grad);
 void launch_count_eval(Ctx &c);
 int poisson_ld(const Ctx &c);
+// ---- controller.hip ----
+struct CtrlDev;
+void launch_ctrl_step(Ctx &c, const CtrlDev *ctrl, int active);
+const float *ctrl_params(const CtrlDev *ctrl);
+const CtrlDev *ctrl_of(const ep_ctrl *x);
+int ctrl_device(const CtrlDev *ctrl);
 
 #define EP_HIP(call)                                                           \
   do {                                                                         \

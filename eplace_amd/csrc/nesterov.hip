@@ -45,7 +45,11 @@ __global__ void __launch_bounds__(kBlock) step_scalar_kernel(int nblocks, double
   if (sc->done) return;
   double num = block_sum_partials_strided(partials + kPartialLip, nblocks, 2);
   double den = block_sum_partials_strided(partials + kPartialLip + 1, nblocks, 2);
+  double nw = block_sum_partials_strided(partials + kPartialNorm, nblocks, 2);
+  double nd = block_sum_partials_strided(partials + kPartialNorm + 1, nblocks, 2);
   if (threadIdx.x == 0) {
+    sc->gwl_norm2 = nw;
+    sc->gd_norm2 = nd;
     double step = 1.0;
     if (sc->iter > 0) {
       double L = sqrt(num) / sqrt(den);
@@ -80,7 +84,11 @@ __global__ void __launch_bounds__(kBlock) accept_kernel(int nblocks, double *par
     }
     int ok = 0.95 * sc->step <= ns;
     sc->accepted = ok;
-    if (!ok) sc->backtracks += 1;
+    if (!ok) {
+      sc->backtracks += 1;
+      if (sc->trial == 0 && sc->ctrl_active) sc->fallbacks += 1;  // controller step rejected
+    }
+    sc->trial += 1;
     sc->step = ns;
   }
 }
@@ -116,19 +124,29 @@ __global__ void done_check_kernel(DevScalars *sc, int next_iter, double *log, in
   if (sc->done) return;
   int k = next_iter - 1;
   if (log && k >= 0 && k < log_cap) {
-    double *r = log + (size_t)k * 8;
-    r[0] = k;
-    r[1] = sc->overflow;
-    r[2] = sc->lambda;
-    r[3] = sc->hpwl;
-    r[4] = sc->step;
-    r[5] = sc->inv_gamma_x;
-    r[6] = sc->evals;
-    r[7] = sc->backtracks;
+    double *r = log + (size_t)k * EP_LOG_COLS;
+    r[EP_LOG_ITER] = k;
+    r[EP_LOG_TAU] = sc->overflow;
+    r[EP_LOG_LAMBDA] = sc->lambda;
+    r[EP_LOG_HPWL] = sc->hpwl;
+    r[EP_LOG_STEP] = sc->step;
+    r[EP_LOG_INV_GAMMA] = sc->inv_gamma_x;
+    r[EP_LOG_EVALS] = sc->evals;
+    r[EP_LOG_BACKTRACKS] = sc->backtracks;
+    for (int i = 0; i < EP_CTRL_FEATURES; ++i) r[EP_LOG_FEATURES + i] = sc->feat[i];
+    r[EP_LOG_STEP_PROP] = sc->step_prop;
+    r[EP_LOG_STEP_USED] = sc->step_used;
+    r[EP_LOG_LAMBDA_BEFORE] = sc->lambda_before;
+    r[EP_LOG_FALLBACKS] = sc->fallbacks;
   }
   if (set_iter) {
     sc->a_k = sc->a_next;
     sc->iter = next_iter;
+    // accepted step history for the controller features
+    sc->step_hist[2] = sc->step_hist[1];
+    sc->step_hist[1] = sc->step_hist[0];
+    sc->step_hist[0] = sc->step;
+    if (k == 0) sc->step0 = sc->step;
   }
   bool stop;
   if (sc->stage == EP_STAGE_MGP) stop = ((float)sc->overflow < sc->target_overflow) || (next_iter > sc->max_iter);

@@ -253,7 +253,8 @@ __global__ void __launch_bounds__(kBlock) wl_group_kernel(int num_groups, const 
 }
 
 // Sums the per-block HPWL/WL partials in a fixed order; optionally applies
-// updatePenaltyFactor (eplace.cpp:846-883) with FP64 HPWL and lambda.
+// updatePenaltyFactor (eplace.cpp:846-883) with FP64 HPWL and lambda
+// (lambda_update 1), times the controller's factor (lambda_update 2).
 __global__ void __launch_bounds__(kBlock) reduce_wl_kernel(int nslots, double *partials, DevScalars *sc,
                                                             int lambda_update) {
   if (sc->done) return;
@@ -266,6 +267,9 @@ __global__ void __launch_bounds__(kBlock) reduce_wl_kernel(int nslots, double *p
       double delta = hp - sc->last_hpwl;
       double mult = delta < 0.0 ? 1.05 : pow(1.05, -delta / 350000.0 + 1.0);
       mult = fmin(1.05, fmax(0.95, mult));
+      // the controller's factor of this iteration, kept inside the
+      // reference's own multiplier range
+      if (lambda_update == 2) mult = fmin(1.05, fmax(0.95, mult * sc->lam_mult));
       sc->lambda *= mult;
       sc->last_hpwl = hp;
     }

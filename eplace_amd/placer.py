@@ -203,36 +203,50 @@ class EPlacer2D:
 
 
 class EplaceNesterovOpt:
-    """FirstOrderOptimizer<VECTOR_3D>::opt() for one stage, device-resident."""
+    """FirstOrderOptimizer<VECTOR_3D>::opt() for one stage, device-resident.
+
+    method: "vanilla" (default), "bktrk" (-bktrk), or "learned" (backtracking
+    whose first trial step and lambda multiplier come from `controller`).
+    """
+
+    METHODS = {"vanilla": 0, "bktrk": 1, "learned": 2}
 
     def __init__(self, placer: EPlacer2D, method: str = "vanilla", target_overflow: float = 0.10,
-                 max_iterations: int = 1000, max_backtracks: int = 30, sync_every: int = 16):
+                 max_iterations: int = 1000, max_backtracks: int = 30, sync_every: int = 16, controller=None,
+                 log: bool = True):
+        if method not in self.METHODS:
+            raise ValueError(f"method must be one of {sorted(self.METHODS)}")
+        if method == "learned" and controller is None:
+            raise ValueError("the learned method needs a controller")
         self.placer = placer
         self.method = method
         self.target_overflow = target_overflow
         self.max_iterations = max_iterations
         self.max_backtracks = max_backtracks
         self.sync_every = sync_every
+        self.controller = controller
+        self.keep_log = log
         self.log = None
         self.result = None
 
     def opt(self, profile: bool = False) -> dict:
-        cap = self.max_iterations + 2
-        log = np.zeros((cap, 8), np.float64)
+        cap = self.max_iterations + 2 if self.keep_log else 0
+        log = np.zeros((max(cap, 1), L.EP_LOG_COLS), np.float64)
         phases = np.zeros(6, np.float64)
-        p = L.EPOptParams(method=1 if self.method == "bktrk" else 0, target_overflow=self.target_overflow,
+        p = L.EPOptParams(method=self.METHODS[self.method], target_overflow=self.target_overflow,
                           max_iterations=self.max_iterations, max_backtracks=self.max_backtracks,
-                          sync_every=self.sync_every, log_capacity=cap, log=_ptr(log),
-                          phase_seconds=_ptr(phases) if profile else None)
+                          sync_every=self.sync_every, log_capacity=cap, log=_ptr(log) if cap else None,
+                          phase_seconds=_ptr(phases) if profile else None,
+                          controller=self.controller.h if self.controller is not None else None)
         r = L.EPRunStats()
         self.placer._check(self.placer.lib.ep_nesterov_run(self.placer.h, C.byref(p), C.byref(r)))
-        res = {k: getattr(r, k) for k, _ in L.EPRunStats._fields_}
+        res = {k: getattr(r, k) for k, _ in L.EPRunStats._fields_ if k != "pad"}
         if profile:
             n = max(phases[5], 1.0)
             res["phase_seconds"] = dict(zip(["scatter", "finalize", "wirelength", "poisson", "gather_combine"],
                                             (phases[:5] / n).tolist()))
             res["phase_evaluations"] = int(phases[5])
-        self.log = log[: res["iterations"]]
+        self.log = log[: res["iterations"]] if cap else None
         if self.placer.placementStage == MGP:
             self.placer.mGPIterationCount = res["iterations"]
         self.result = res

@@ -1,0 +1,138 @@
+"""Train the step / lambda controller on plain backtracking trajectories and
+evaluate it against plain backtracking on unseen designs.
+
+    python -m eplace_amd.train_controller [--designs 64] [--out eplace_amd/controller_params.npy]
+
+Training designs: seeded synthetic designs of 50K-300K cells in the std-cell
+recipe (lognormal widths, 1K IO, 1.05 nets per cell, utilisation 0.7,
+target density 1.0), every fourth with 20 movable macros; the reference's
+bin rule.  Each is placed by plain backtracking Nesterov with the optimiser
+log on; every logged iteration is a training pair (controller.samples_from_log).
+Evaluation: unseen seeds at 300K cells and 512x512 bins and at 1M cells
+(the reference's bin rule), plain vs learned, reporting iterations, full
+evaluations, wall time, final overflow and the HPWL gap.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import EPlacer2D, EplaceNesterovOpt, synthetic
+from .controller import PARAMS_PATH, Controller, fit, samples_from_log
+
+
+def training_design(i: int, cells: int):
+    macros = 20 if i % 4 == 3 else 0
+    return synthetic(cells, int(round(cells * 1.05)), seed=10_000 + i, num_io=1000, util=0.7,
+                     num_macros=macros, name=f"train{i}")
+
+
+def trajectories(n: int, lo: int, hi: int, poisson_mode: int = 1, verbose: bool = True):
+    """Plain backtracking logs of n training designs."""
+    rng = np.random.default_rng(7)
+    logs, info = [], []
+    for i in range(n):
+        cells = int(rng.integers(lo, hi + 1))
+        d = training_design(i, cells)
+        gp = EPlacer2D(d, 1.0, poisson_mode=poisson_mode)
+        gp.initialization()
+        opt = EplaceNesterovOpt(gp, method="bktrk")
+        r = opt.opt()
+        logs.append(opt.log.copy())
+        info.append({"cells": cells, "iterations": r["iterations"]})
+        if verbose:
+            print(f"[train] design {i}: {cells} cells, {r['iterations']} iterations", flush=True)
+        gp.close()
+    return logs, info
+
+
+def samples(logs, lookahead: float = 1.0):
+    pairs = [samples_from_log(g, lookahead) for g in logs]
+    return np.concatenate([p[0] for p in pairs]), np.concatenate([p[1] for p in pairs])
+
+
+def compare(ctrl: Controller, design, bins=0, poisson_mode: int = 1, target_overflow: float = 0.10):
+    """Plain backtracking vs learned on one design, both from the same start."""
+    out = {}
+    for method in ("bktrk", "learned"):
+        gp = EPlacer2D(design, 1.0, bins=bins, poisson_mode=poisson_mode)
+        gp.initialization()
+        opt = EplaceNesterovOpt(gp, method=method, target_overflow=target_overflow,
+                                controller=ctrl if method == "learned" else None, log=False)
+        t = time.perf_counter()
+        r = opt.opt()
+        r["wall"] = time.perf_counter() - t
+        out[method] = r
+        gp.close()
+    p, q = out["bktrk"], out["learned"]
+    out["speedup"] = p["seconds"] / q["seconds"]
+    out["evaluations_saved"] = 1.0 - q["evaluations"] / p["evaluations"]
+    out["hpwl_gap"] = q["final_hpwl"] / p["final_hpwl"] - 1.0
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--designs", type=int, default=64)
+    ap.add_argument("--min-cells", type=int, default=50_000)
+    ap.add_argument("--max-cells", type=int, default=300_000)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--lookahead", type=float, default=1.5,
+                    help="penalty growth target: plain's over this many iterations per iteration")
+    ap.add_argument("--eval-seeds", type=int, default=8)
+    ap.add_argument("--eval-cells", type=int, default=300_000)
+    ap.add_argument("--metric-seeds", type=int, default=3, help="also evaluate 1M-cell designs of these seeds")
+    ap.add_argument("--out", default=PARAMS_PATH)
+    ap.add_argument("--report", default=None, help="write the evaluation as JSON here")
+    a = ap.parse_args(argv)
+    t = time.perf_counter()
+    logs, info = trajectories(a.designs, a.min_cells, a.max_cells)
+    X, Y = samples(logs, a.lookahead)
+    print(f"[train] {len(X)} samples from {a.designs} designs in {time.perf_counter() - t:.1f}s", flush=True)
+    t = time.perf_counter()
+    ctrl, losses = fit(X, Y, steps=a.steps, lr=a.lr)
+    print(f"[train] {a.steps} Adam steps in {time.perf_counter() - t:.1f}s: loss {losses[0]:.4f} -> {losses[-1]:.4f}",
+          flush=True)
+    ctrl.save(a.out)
+    def run_set(tag, cases):
+        rows = []
+        for s, d, bins in cases:
+            r = compare(ctrl, d, bins=bins)
+            rows.append(r)
+            p, q = r["bktrk"], r["learned"]
+            print(f"[{tag}] seed {s}: plain {p['iterations']} it / {p['evaluations']} ev / {p['seconds']:.3f}s, "
+                  f"learned {q['iterations']} it / {q['evaluations']} ev / {q['seconds']:.3f}s "
+                  f"({q['controller_fallbacks']} fallbacks), speedup {r['speedup']:.2f}, "
+                  f"HPWL gap {100 * r['hpwl_gap']:+.3f}%, overflow {q['final_overflow']:.4f}", flush=True)
+        out = {"runs": [{"plain": r["bktrk"], "learned": r["learned"], "speedup": r["speedup"],
+                         "evaluations_saved": r["evaluations_saved"], "hpwl_gap": r["hpwl_gap"]} for r in rows]}
+        if rows:
+            out.update(mean_speedup=float(np.mean([r["speedup"] for r in rows])),
+                       mean_hpwl_gap=float(np.mean([r["hpwl_gap"] for r in rows])),
+                       max_hpwl_gap=float(np.max([r["hpwl_gap"] for r in rows])))
+            print(f"[{tag}] mean speedup {out['mean_speedup']:.2f}, mean HPWL gap {100 * out['mean_hpwl_gap']:+.3f}%, "
+                  f"max {100 * out['max_hpwl_gap']:+.3f}%", flush=True)
+        return out
+
+    small = [(s, synthetic(a.eval_cells, int(round(a.eval_cells * 1.05)), seed=90_000 + s, num_io=1000, util=0.7,
+                           name=f"eval{s}"), 512) for s in range(a.eval_seeds)]
+    ev = run_set("eval", small)
+    del small
+    big = [(s, synthetic(1_000_000, 1_050_000, seed=2024 + s, num_io=1000, util=0.7, name=f"m{s}"), 0)
+           for s in range(a.metric_seeds)]
+    mv = run_set("1M", big)
+    summary = {"samples": int(len(X)), "lookahead": a.lookahead, "designs": info, "loss_first": float(losses[0]),
+               "loss_last": float(losses[-1]), "eval_300k_512bins": ev, "eval_1m": mv}
+    if a.report:
+        with open(a.report, "w") as f:
+            json.dump(summary, f, indent=1)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -22,6 +22,7 @@ extern "C" {
 #endif
 
 typedef struct ep_ctx ep_ctx;
+typedef struct ep_ctrl ep_ctrl;
 
 enum {
   EP_OK = 0,
@@ -85,18 +86,42 @@ typedef struct ep_stats {
   int32_t mgp_iterations;
 } ep_stats;
 
+/* Columns of one optimiser log row (written at the end of iteration k). */
+enum {
+  EP_LOG_ITER = 0,          /* k */
+  EP_LOG_TAU = 1,           /* overflow at the accepted point y_{k+1} */
+  EP_LOG_LAMBDA = 2,        /* lambda after the update of iteration k */
+  EP_LOG_HPWL = 3,          /* HPWL at y_{k+1} */
+  EP_LOG_STEP = 4,          /* 1/L' measured at the accepted trial (backtracking) */
+  EP_LOG_INV_GAMMA = 5,     /* 1/gamma_x */
+  EP_LOG_EVALS = 6,         /* full evaluations so far */
+  EP_LOG_BACKTRACKS = 7,    /* rejected trials so far */
+  EP_LOG_FEATURES = 8,      /* EP_CTRL_FEATURES controller inputs at y_k (backtracking/learned) */
+  EP_LOG_STEP_PROP = 20,    /* plain step proposal 1/L at iteration k */
+  EP_LOG_STEP_USED = 21,    /* step of the first trial (the controller's under method 2) */
+  EP_LOG_LAMBDA_BEFORE = 22,/* lambda used by the evaluations of iteration k */
+  EP_LOG_FALLBACKS = 23,    /* controller steps rejected so far */
+  EP_LOG_COLS = 24
+};
+
 /* Optimiser settings (nesterov.hpp:86-149, 231-415). */
 typedef struct ep_opt_params {
-  int32_t method;           /* 0 vanilla (default), 1 backtracking (-bktrk) */
+  int32_t method;           /* 0 vanilla (default), 1 backtracking (-bktrk),
+                               2 learned: backtracking whose first trial step and
+                               a factor on the penalty multiplier come from
+                               `controller`; a first trial that fails the
+                               backtracking acceptance test continues as plain
+                               backtracking */
   float target_overflow;    /* mGP stop threshold (default 0.10) */
   int32_t max_iterations;   /* MAX_ITERATION (1000); stop when iter > this */
   int32_t max_backtracks;   /* cap on step retries per iteration (reference: none) */
   int32_t sync_every;       /* host polls the stop flag every this many iterations */
   int32_t log_capacity;     /* rows available in log (may be 0) */
-  double *log;              /* optional [log_capacity][8]: iter, tau, lambda, hpwl, step, invgx, evals, backtracks */
+  double *log;              /* optional [log_capacity][EP_LOG_COLS], columns EP_LOG_* */
   double *phase_seconds;    /* optional [6]: summed kernel time of the full evaluations per phase
                                (scatter, density finalise, wirelength, poisson, gather+combine) and
                                the number of evaluations timed; HIP events on the engine's stream */
+  const ep_ctrl *controller; /* method 2: the step / lambda controller (same device) */
 } ep_opt_params;
 
 typedef struct ep_run_stats {
@@ -107,6 +132,8 @@ typedef struct ep_run_stats {
   double final_overflow;
   double final_hpwl;        /* HPWL after wrap_up (main solution written back) */
   double seconds;           /* wall time of the loop */
+  int32_t controller_fallbacks; /* method 2: first trials that failed the acceptance test */
+  int32_t pad;
 } ep_run_stats;
 
 /* Which device array ep_read/ep_write address (parity and debugging). */
@@ -171,6 +198,48 @@ int ep_write(ep_ctx *ctx, int32_t which, const void *host, int64_t count);
 /* Wall time in seconds of the last ep_total_gradient_update's kernels, split
  * per phase: [scatter, density finalise, wirelength, poisson, gather+combine]. */
 int ep_last_timings(ep_ctx *ctx, double *seconds5);
+
+/* ---------------------------------------------------------------------------
+ * Learned step / lambda controller.  No counterpart in the reference: an MLP
+ * EP_CTRL_FEATURES -> EP_CTRL_HIDDEN -> EP_CTRL_HIDDEN -> EP_CTRL_OUTPUTS
+ * (tanh hidden layers, FP32) evaluated on the device once per iteration of
+ * method 2.  Inputs are the EP_LOG_FEATURES columns; outputs are
+ * log(step / plain proposal), clamped to [log 1/4, 0], and the log of a
+ * factor on the reference's penalty multiplier (eplace.cpp:846-883),
+ * clamped to [-0.05, 0.1], the product kept in the reference's range
+ * [0.95, 1.05]; zero outputs reproduce plain backtracking.
+ *
+ * Parameter vector (EP_CTRL_PARAMS floats): W1[H][F], b1[H], W2[H][H], b2[H],
+ * W3[O][H], b3[O] (the EP_CTRL_WEIGHTS trained values), then the input
+ * normalisation mean[F], std[F] and output mean[O], std[O]: the network sees
+ * (x - mean) / std and its output is scaled back by std and mean.
+ * ------------------------------------------------------------------------ */
+enum {
+  EP_CTRL_FEATURES = 12,
+  EP_CTRL_HIDDEN = 64,
+  EP_CTRL_OUTPUTS = 2,
+  EP_CTRL_WEIGHTS = 12 * 64 + 64 + 64 * 64 + 64 + 2 * 64 + 2,
+  EP_CTRL_PARAMS = 12 * 64 + 64 + 64 * 64 + 64 + 2 * 64 + 2 + 2 * 12 + 2 * 2
+};
+
+/* params: EP_CTRL_PARAMS floats, or NULL for a seeded initialisation with
+ * identity normalisation. */
+int ep_ctrl_create(int32_t device, const float *params, uint32_t seed, ep_ctrl **out);
+int ep_ctrl_destroy(ep_ctrl *ctrl);
+const char *ep_ctrl_last_error(const ep_ctrl *ctrl);
+int ep_ctrl_get_params(ep_ctrl *ctrl, float *params);
+/* Replaces all parameters and resets the optimiser state. */
+int ep_ctrl_set_params(ep_ctrl *ctrl, const float *params);
+/* Full-batch Adam on the mean squared error of the normalised outputs over
+ * n samples (X [n][F], Y [n][O], host arrays), `steps` updates at learning
+ * rate lr; loss_out (optional, [steps]) receives the loss before each update. */
+int ep_ctrl_train(ep_ctrl *ctrl, const float *X, const float *Y, int64_t n, int32_t steps, float lr,
+                  float *loss_out);
+/* Y [n][O] = controller outputs for X [n][F]. */
+int ep_ctrl_predict(ep_ctrl *ctrl, const float *X, int64_t n, float *Y);
+/* Loss and its gradient with respect to the EP_CTRL_WEIGHTS weights at the
+ * current parameters (the training kernels' backward pass). */
+int ep_ctrl_loss_grad(ep_ctrl *ctrl, const float *X, const float *Y, int64_t n, float *grad, float *loss);
 
 #ifdef __cplusplus
 }
