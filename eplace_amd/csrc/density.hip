@@ -96,7 +96,7 @@ __global__ void __launch_bounds__(kBlock) tile_key_kernel(int O, const float2 *_
                                                            const float *__restrict__ h,
                                                            const uint8_t *__restrict__ kind,
                                                            const uint8_t *__restrict__ is_big, Grid g, int tyc,
-                                                           uint32_t *keys, float4 *vals, int *ranges,
+                                                           uint64_t *keys, float4 *vals, int *ranges,
                                                            int nranges) {
   int o = blockIdx.x * blockDim.x + threadIdx.x;
   if (o < nranges) ranges[o] = 0;
@@ -109,18 +109,18 @@ __global__ void __launch_bounds__(kBlock) tile_key_kernel(int O, const float2 *_
     int ti = min(f.i0, g.nx - 1) / kTile, tj = min(f.j0, g.ny - 1) / kTile;
     key = (uint32_t)kind[o] * kTileKeyStride + (uint32_t)(ti * tyc + tj);
   }
-  keys[o] = key;
+  keys[o] = (uint64_t)key << 32 | (uint32_t)o;
   vals[o] = make_float4(cx, cy, ww, hh);
 }
 
 // begin/end of each key's run in the sorted order (ranges[2k], ranges[2k+1]).
-__global__ void tile_range_kernel(int O, const uint32_t *__restrict__ keys, int *ranges) {
+__global__ void tile_range_kernel(int O, const uint64_t *__restrict__ keys, int *ranges) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= O) return;
-  uint32_t k = keys[i];
+  uint32_t k = (uint32_t)(keys[i] >> 32);
   if (k >= kTileKeySentinel) return;
-  if (i == 0 || keys[i - 1] != k) ranges[2 * k] = i;
-  if (i == O - 1 || keys[i + 1] != k) ranges[2 * k + 1] = i + 1;
+  if (i == 0 || (uint32_t)(keys[i - 1] >> 32) != k) ranges[2 * k] = i;
+  if (i == O - 1 || (uint32_t)(keys[i + 1] >> 32) != k) ranges[2 * k + 1] = i + 1;
 }
 
 // One block: chunks of <= kTileChunk objects per key run, as (key, begin, end).
@@ -248,7 +248,7 @@ __global__ void __launch_bounds__(kBlock) finalize_density_kernel(
 
 // tau = overflow area / node area, mean rho, and the gamma schedule
 // (eplace.cpp:475-500) in the reference's float arithmetic.
-__global__ void __launch_bounds__(kBlock) finish_density_kernel(int nblocks, int nb, Grid g,
+__global__ void __launch_bounds__(kFinish) finish_density_kernel(int nblocks, int nb, Grid g,
                                                                  double node_area_scaled, const double *partials,
                                                                  DevScalars *sc) {
   if (sc->done) return;
@@ -286,6 +286,64 @@ __device__ __forceinline__ float2 gather_one(const Foot &f, const Grid &g, int l
   return make_float2(a * ex[k], a * ey[k]);
 }
 
+// Tile-sorted gather: the chunks of the last scatter (objects of one kind
+// whose footprint starts in one 32x32-bin tile) read Ex/Ey from an LDS copy
+// of the tile plus its high-side halo; footprint bins outside it come from
+// global memory.  Same arithmetic and order as the per-object gather, so
+// the result is identical.
+__global__ void __launch_bounds__(kBlock) gather_tile_kernel(const int4 *__restrict__ chunks,
+                                                              const int *__restrict__ nchunks,
+                                                              const uint64_t *__restrict__ keys,
+                                                              const float4 *__restrict__ vals, Grid g, int tyc,
+                                                              int ldp, const float *__restrict__ ex,
+                                                              const float *__restrict__ ey, float2 *gd,
+                                                              const DevScalars *sc) {
+  if (sc->done) return;
+  __shared__ float wex[kTileWin * kTileWin], wey[kTileWin * kTileWin];
+  const int total = *nchunks;
+  for (int ci = blockIdx.x; ci < total; ci += gridDim.x) {
+    const int4 ch = chunks[ci];
+    const int tile = ch.x % kTileKeyStride;
+    const int ib = (tile / tyc) * kTile, jb = (tile % tyc) * kTile;
+    __syncthreads();  // the previous chunk is done with the window
+    for (int t = threadIdx.x; t < kTileWin * kTileWin; t += blockDim.x) {
+      const int i = ib + t / kTileWin, j = jb + t % kTileWin;
+      float a = 0.f, b = 0.f;
+      if (i < g.nx && j < g.ny) {
+        const size_t k = (size_t)i * ldp + j;
+        a = ex[k];
+        b = ey[k];
+      }
+      wex[t] = a;
+      wey[t] = b;
+    }
+    __syncthreads();
+    for (int q = ch.y + threadIdx.x; q < ch.z; q += blockDim.x) {
+      const float4 v = vals[q];
+      const Foot f = footprint(v.x, v.y, v.z, v.w, g);
+      float gx = 0.f, gy = 0.f;
+      for (int i = f.i0; i <= f.i1; ++i)
+        for (int j = f.j0; j <= f.j1; ++j) {
+          const double ov = bin_overlap(g, i, j, f);
+          const float a = (float)((double)(f.sx * f.sy) * ov);
+          const int li = i - ib, lj = j - jb;
+          float e0, e1;
+          if (li < kTileWin && lj < kTileWin) {
+            e0 = wex[li * kTileWin + lj];
+            e1 = wey[li * kTileWin + lj];
+          } else {
+            const size_t k = (size_t)i * ldp + j;
+            e0 = ex[k];
+            e1 = ey[k];
+          }
+          gx += a * e0;
+          gy += a * e1;
+        }
+      gd[(uint32_t)keys[q]] = make_float2(gx, gy);
+    }
+  }
+}
+
 __global__ void __launch_bounds__(kBlock) gather_big_kernel(const int32_t *__restrict__ big,
                                                              const float2 *__restrict__ pos,
                                                              const float *__restrict__ w,
@@ -311,8 +369,8 @@ __global__ void __launch_bounds__(kBlock) gather_big_kernel(const int32_t *__res
   if (threadIdx.x == 0) gd[o] = make_float2((float)gx, (float)gy);
 }
 
-// Gather (small objects), node wirelength-gradient reduction over the
-// node's pins in modulePins order (eplace.cpp:531-549), preconditioned
+// Node wirelength-gradient reduction over the node's pins in modulePins
+// order (eplace.cpp:531-549) when recompute is set, preconditioned
 // combine (eplace.cpp:694-737) and, optionally, the Lipschitz partial sums
 // |g - g_prev|^2 and |y - y_prev|^2 over the stage's objects
 // (optutils.hpp:19-35) with, for the step controller, |grad WL|^2 and
@@ -320,11 +378,9 @@ __global__ void __launch_bounds__(kBlock) gather_big_kernel(const int32_t *__res
 // nets stored by their lowest node, the pin gradients a range of nodes
 // reads lie in a narrow band of the pin array.
 __global__ void __launch_bounds__(kBlock) gather_combine_kernel(
-    int O, int N, const float2 *__restrict__ pos, const float *__restrict__ w,
-    const float *__restrict__ h, const uint8_t *__restrict__ kind, const uint8_t *__restrict__ is_big,
-    const int32_t *__restrict__ npins, const float *__restrict__ area, Grid g, int ldp,
-    const float *__restrict__ ex, const float *__restrict__ ey, const int32_t *__restrict__ node_pin_start,
-    const int32_t *__restrict__ node_pins, const float2 *__restrict__ pin_grad, float2 *gd, float2 *gwl,
+    int O, int N, const float2 *__restrict__ pos, const uint8_t *__restrict__ kind,
+    const int32_t *__restrict__ npins, const float *__restrict__ area, const int32_t *__restrict__ node_pin_start,
+    const float2 *__restrict__ pin_grad, const float2 *gd, float2 *gwl,
     float2 *gtot, int recompute, int lipschitz, const uint8_t *__restrict__ in_stage,
     const float2 *__restrict__ ref_now, const float2 *__restrict__ ref_prev,
     const float2 *__restrict__ grad_prev, double *partials, const DevScalars *sc) {
@@ -333,28 +389,13 @@ __global__ void __launch_bounds__(kBlock) gather_combine_kernel(
   double ln = 0, ld = 0, nw = 0, nd = 0;
   if (o < O) {
     uint8_t k = kind[o];
-    float2 d;
-    if (recompute && !is_big[o]) {
-      const float2 p = pos[o];
-      Foot f = footprint(p.x, p.y, w[o], h[o], g);
-      float gx = 0, gy = 0;
-      for (int i = f.i0; i <= f.i1; ++i)
-        for (int j = f.j0; j <= f.j1; ++j) {
-          float2 v = gather_one(f, g, ldp, ex, ey, i, j);
-          gx += v.x;
-          gy += v.y;
-        }
-      d = make_float2(gx, gy);
-      gd[o] = d;
-    } else {
-      d = gd[o];
-    }
+    const float2 d = gd[o];  // from the tile / big-object gather
     float2 wl = make_float2(0.f, 0.f);
     if (o < N) {
       if (recompute) {
         int s = node_pin_start[o], e = node_pin_start[o + 1];
         for (int q = s; q < e; ++q) {
-          float2 pg = pin_grad[node_pins[q]];
+          float2 pg = pin_grad[q];
           wl.x += pg.x;
           wl.y += pg.y;
         }
@@ -423,7 +464,7 @@ __global__ void __launch_bounds__(kBlock) lambda_init_kernel(int O, int N, const
   }
 }
 
-__global__ void __launch_bounds__(kBlock) lambda_init_finish_kernel(int nblocks, const double *partials,
+__global__ void __launch_bounds__(kFinish) lambda_init_finish_kernel(int nblocks, const double *partials,
                                                                      DevScalars *sc) {
   double na = block_sum_partials_strided(partials + kPartialLambda, nblocks, 2);
   double nb = block_sum_partials_strided(partials + kPartialLambda + 1, nblocks, 2);
@@ -449,10 +490,12 @@ void launch_scatter(Ctx &c) {
   const int nranges = 2 * kTileKeySentinel;
   hipLaunchKernelGGL(tile_key_kernel, dim3(blocks_for(std::max(c.O, nranges))), dim3(kBlock), 0, c.stream, c.O,
                      c.pos, c.w, c.h, c.kind, c.is_big, g, t.tyc, t.keys[0], t.vals[0], t.ranges, nranges);
-  hipcub::DoubleBuffer<uint32_t> kb(t.keys[0], t.keys[1]);
+  hipcub::DoubleBuffer<uint64_t> kb(t.keys[0], t.keys[1]);
   hipcub::DoubleBuffer<float4> vb(t.vals[0], t.vals[1]);
   size_t bytes = t.temp_bytes;
-  (void)hipcub::DeviceRadixSort::SortPairs(t.temp, bytes, kb, vb, c.O, 0, kTileKeyBits, c.stream);
+  (void)hipcub::DeviceRadixSort::SortPairs(t.temp, bytes, kb, vb, c.O, 32, 32 + kTileKeyBits, c.stream);
+  t.skeys = kb.Current();
+  t.svals = vb.Current();
   hipLaunchKernelGGL(tile_range_kernel, dim3(blocks_for(c.O)), dim3(kBlock), 0, c.stream, c.O, kb.Current(),
                      t.ranges);
   hipLaunchKernelGGL(tile_chunk_kernel, dim3(1), dim3(1024), 0, c.stream, kTileKeySentinel, t.ranges, t.chunks,
@@ -475,17 +518,18 @@ bool tile_sort_create(Ctx &c, std::string &err) {
     return false;
   }
   size_t n = std::max(c.O, 1);
-  bool ok = hipMalloc(&t.keys[0], n * 4) == hipSuccess && hipMalloc(&t.keys[1], n * 4) == hipSuccess &&
+  bool ok = hipMalloc(&t.keys[0], n * 8) == hipSuccess && hipMalloc(&t.keys[1], n * 8) == hipSuccess &&
             hipMalloc(&t.vals[0], n * 16) == hipSuccess && hipMalloc(&t.vals[1], n * 16) == hipSuccess &&
             hipMalloc(&t.ranges, 2 * kTileKeySentinel * sizeof(int)) == hipSuccess &&
             hipMalloc(&t.nchunks, sizeof(int)) == hipSuccess;
   size_t max_chunks = n / kTileChunk + 3 * (size_t)kTileKeyStride + 1;
   ok = ok && hipMalloc(&t.chunks, max_chunks * sizeof(int4)) == hipSuccess;
   if (ok) {
-    hipcub::DoubleBuffer<uint32_t> kb(t.keys[0], t.keys[1]);
+    hipcub::DoubleBuffer<uint64_t> kb(t.keys[0], t.keys[1]);
     hipcub::DoubleBuffer<float4> vb(t.vals[0], t.vals[1]);
     t.temp_bytes = 0;
-    ok = hipcub::DeviceRadixSort::SortPairs(nullptr, t.temp_bytes, kb, vb, (int)n, 0, kTileKeyBits) == hipSuccess &&
+    ok = hipcub::DeviceRadixSort::SortPairs(nullptr, t.temp_bytes, kb, vb, (int)n, 32, 32 + kTileKeyBits) ==
+             hipSuccess &&
          hipMalloc(&t.temp, std::max<size_t>(t.temp_bytes, 16)) == hipSuccess;
   }
   t.grid = (int)std::min<size_t>(max_chunks, 2048);
@@ -508,7 +552,7 @@ void launch_finalize_density(Ctx &c) {
   hipLaunchKernelGGL(finalize_density_kernel, dim3(blocks_for(nb)), dim3(kBlock), 0, c.stream, nb, c.node_q,
                      c.filler_q, c.term_map, c.base_map, g, poisson_ld(c), ldexp(1.0, -c.shift), c.rho,
                      c.partials, c.sc);
-  hipLaunchKernelGGL(finish_density_kernel, dim3(1), dim3(kBlock), 0, c.stream, blocks_for(nb), nb, g, nas,
+  hipLaunchKernelGGL(finish_density_kernel, dim3(1), dim3(kFinish), 0, c.stream, blocks_for(nb), nb, g, nas,
                      c.partials, c.sc);
 }
 
@@ -516,19 +560,24 @@ void launch_gather_combine(Ctx &c, bool recompute, bool lipschitz, const float2 
                            const float2 *ref_prev, const float2 *grad_prev) {
   Grid g = c.grid();
   int ldp = poisson_ld(c);
-  if (recompute && c.num_big)
-    hipLaunchKernelGGL(gather_big_kernel, dim3(c.num_big), dim3(kBlock), 0, c.stream, c.big, c.pos, c.w, c.h,
-                       g, ldp, c.ex, c.ey, c.gd, c.sc);
+  if (recompute) {
+    TileSort &t = c.tiles;
+    hipLaunchKernelGGL(gather_tile_kernel, dim3(t.grid), dim3(kBlock), 0, c.stream, t.chunks, t.nchunks, t.skeys,
+                       t.svals, g, t.tyc, ldp, c.ex, c.ey, c.gd, c.sc);
+    if (c.num_big)
+      hipLaunchKernelGGL(gather_big_kernel, dim3(c.num_big), dim3(kBlock), 0, c.stream, c.big, c.pos, c.w, c.h,
+                         g, ldp, c.ex, c.ey, c.gd, c.sc);
+  }
   hipLaunchKernelGGL(gather_combine_kernel, dim3(object_blocks(c.O)), dim3(kBlock), 0, c.stream, c.O, c.N,
-                     c.pos, c.w, c.h, c.kind, c.is_big, c.npins, c.area, g, ldp, c.ex, c.ey, c.node_pin_start,
-                     c.node_pins, c.pin_grad, c.gd, c.gwl, c.gtot, recompute ? 1 : 0, lipschitz ? 1 : 0,
-                     c.in_stage, ref_now, ref_prev, grad_prev, c.partials, c.sc);
+                     c.pos, c.kind, c.npins, c.area, c.node_pin_start, c.pin_grad, c.gd, c.gwl,
+                     c.gtot, recompute ? 1 : 0, lipschitz ? 1 : 0, c.in_stage, ref_now, ref_prev, grad_prev,
+                     c.partials, c.sc);
 }
 
 void launch_lambda_init(Ctx &c) {
   hipLaunchKernelGGL(lambda_init_kernel, dim3(blocks_for(c.O)), dim3(kBlock), 0, c.stream, c.O, c.N, c.gd,
                      c.gwl, c.partials);
-  hipLaunchKernelGGL(lambda_init_finish_kernel, dim3(1), dim3(kBlock), 0, c.stream, blocks_for(c.O), c.partials,
+  hipLaunchKernelGGL(lambda_init_finish_kernel, dim3(1), dim3(kFinish), 0, c.stream, blocks_for(c.O), c.partials,
                      c.sc);
 }
 

@@ -379,7 +379,7 @@ int create(Ctx &c, const ep_design *d, const ep_params *p) {
   }
   std::vector<int32_t> ns32(c.E + 1), pobj(std::max<int64_t>(c.P, 1)), pin_net(std::max<int64_t>(c.P, 1));
   std::vector<float> pdx(std::max<int64_t>(c.P, 1)), pdy(std::max<int64_t>(c.P, 1));
-  c.h_pin_of.assign(c.P, 0);
+  std::vector<int32_t> pin_of(std::max<int64_t>(c.P, 1));  // device pin of each design pin
   {
     int32_t q2 = 0;
     for (int e2 = 0; e2 < c.E; ++e2) {
@@ -391,7 +391,7 @@ int create(Ctx &c, const ep_design *d, const ep_params *p) {
         pdx[q2] = d->pin_dx[q];
         pdy[q2] = d->pin_dy[q];
         pin_net[q2] = e2;
-        c.h_pin_of[q] = q2;
+        pin_of[q] = q2;
       }
     }
     ns32[c.E] = q2;
@@ -417,13 +417,19 @@ int create(Ctx &c, const ep_design *d, const ep_params *p) {
   }
   c.num_groups = (int)gbeg.size();
   c.num_large = (int)large.size();
-  // ---- node -> pins CSR in modulePins (file) order, as device pin indices ----
-  std::vector<int32_t> nps(c.N + 1, 0), npl(std::max<int64_t>(c.P, 1));
+  // ---- pin gradient slots: node by node, each node's pins in modulePins
+  // (file) order, so the node sum reads one contiguous run ----
+  std::vector<int32_t> nps(c.N + 1, 0), slot(std::max<int64_t>(c.P, 1), -1);
   for (int i = 0; i < c.N; ++i) nps[i + 1] = nps[i] + s.npins[i];
+  c.h_grad_slot.assign(c.P, -1);
   {
     std::vector<int32_t> fill(nps.begin(), nps.end() - 1);
     for (int64_t q = 0; q < c.P; ++q)
-      if (d->pin_obj[q] < c.N) npl[fill[d->pin_obj[q]]++] = c.h_pin_of[q];
+      if (d->pin_obj[q] < c.N) {
+        const int32_t sl = fill[d->pin_obj[q]]++;
+        slot[pin_of[q]] = sl;
+        c.h_grad_slot[q] = sl;
+      }
   }
   c.h_w = s.w;
   c.h_h = s.h;
@@ -443,7 +449,7 @@ int create(Ctx &c, const ep_design *d, const ep_params *p) {
             dev_upload(c, c.pin_net, pin_net.data(), pin_net.size()) &&
             dev_upload(c, c.large_nets, large.data(), large.size()) &&
             dev_upload(c, c.node_pin_start, nps.data(), c.N + 1) &&
-            dev_upload(c, c.node_pins, npl.data(), npl.size()) && dev_zero(c, c.pin_grad, c.P) &&
+            dev_upload(c, c.pin_slot, slot.data(), slot.size()) && dev_zero(c, c.pin_grad, (size_t)nps[c.N]) &&
             dev_upload(c, c.bllx, s.bllx.data(), c.nx) && dev_upload(c, c.burx, s.burx.data(), c.nx) &&
             dev_upload(c, c.blly, s.blly.data(), c.ny) && dev_upload(c, c.bury, s.bury.data(), c.ny) &&
             dev_zero(c, c.node_q, nb) && dev_zero(c, c.filler_q, nb) &&
@@ -476,7 +482,7 @@ void destroy(Ctx &c) {
   tile_sort_destroy(c);
   void *ptrs[] = {c.pos, c.w, c.h, c.area, c.kind, c.in_stage, c.npins, c.big, c.is_big,
                   c.net_start, c.pin_obj, c.pin_dx, c.pin_dy, c.group_begin, c.group_end, c.pin_net, c.large_nets, c.node_pin_start,
-                  c.node_pins, c.pin_grad, c.bllx, c.burx, c.blly, c.bury, c.node_q, c.filler_q, c.term_map,
+                  c.pin_slot, c.pin_grad, c.bllx, c.burx, c.blly, c.bury, c.node_q, c.filler_q, c.term_map,
                   c.base_map, c.rho, c.phi, c.ex, c.ey, c.gd, c.gwl, c.gtot, c.cur_main, c.cur_ref, c.last_ref,
                   c.last_grad, c.new_main, c.new_ref, c.cur_grad, c.sc, c.partials, c.tickets};
   for (void *p : ptrs)
@@ -969,11 +975,14 @@ int ep_read(ep_ctx *x, int32_t which, void *host, int64_t count) {
     case EP_ARR_GRAD_TOTAL: return plain(c.gtot, 2 * (int64_t)c.O, 4);
     case EP_ARR_TERM_MAP: return plain(c.term_map, nb, 4);
     case EP_ARR_BASE_MAP: return plain(c.base_map, nb, 4);
-    case EP_ARR_PIN_GRAD: {  // back to design pin order
+    case EP_ARR_PIN_GRAD: {  // back to design pin order; terminal pins read 0
       if (count != 2 * c.P) return c.err = "count mismatch", EP_ERR_ARG;
-      std::vector<float2> g(std::max<int64_t>(c.P, 1));
-      EP_HIP(hipMemcpy(g.data(), c.pin_grad, (size_t)c.P * sizeof(float2), hipMemcpyDeviceToHost));
-      for (int64_t q = 0; q < c.P; ++q) ((float2 *)host)[q] = g[c.h_pin_of[q]];
+      int64_t n = 0;
+      for (int32_t sl : c.h_grad_slot) n += sl >= 0;
+      std::vector<float2> g(std::max<int64_t>(n, 1));
+      EP_HIP(hipMemcpy(g.data(), c.pin_grad, (size_t)n * sizeof(float2), hipMemcpyDeviceToHost));
+      for (int64_t q = 0; q < c.P; ++q)
+        ((float2 *)host)[q] = c.h_grad_slot[q] >= 0 ? g[c.h_grad_slot[q]] : make_float2(0.f, 0.f);
       return EP_OK;
     }
     case EP_ARR_MAIN: return plain(c.cur_main, 2 * (int64_t)c.O, 4);

@@ -15,6 +15,7 @@ enum ObjKind : uint8_t { KIND_CELL = 0, KIND_MACRO = 1, KIND_FILLER = 2 };
 
 constexpr double kEps = 1.0e-15;  // global.h:39
 constexpr int kBlock = 256;
+constexpr int kFinish = 1024;  // single-block kernels that sum per-block partials
 constexpr int kSmallNetMaxDegree = 64;  // nets above this get a wave each; the rest are packed 64 pins per wave
 constexpr int kBigObjectBins = 16;      // objects that may cover more bins get a block each
 constexpr int kXcds = 8;                // MI355X: workgroups go to the 8 XCDs round-robin
@@ -82,15 +83,18 @@ constexpr int kPartialTotal = 10 * kMaxPartials;
 
 struct Poisson;  // poisson.hip
 
-// Tile-sorted scatter state (density.hip).  Keys are kind * kTileKeyStride +
-// tile; kTileKeySentinel marks objects handled by the block-per-object path.
+// Tile-sorted scatter / gather state (density.hip).  Keys are kind *
+// kTileKeyStride + tile in the high 32 bits and the object in the low 32;
+// kTileKeySentinel marks objects handled by the block-per-object path.
 constexpr uint32_t kTileKeyStride = 4096;
 constexpr uint32_t kTileKeySentinel = 3 * kTileKeyStride;
 constexpr int kTileKeyBits = 14;
 struct TileSort {
   int tyc = 0, grid = 0;
-  uint32_t *keys[2] = {nullptr, nullptr};
-  float4 *vals[2] = {nullptr, nullptr};
+  uint64_t *keys[2] = {nullptr, nullptr};  // key << 32 | object
+  float4 *vals[2] = {nullptr, nullptr};    // centre x, y, width, height
+  const uint64_t *skeys = nullptr;         // sorted by the last scatter
+  const float4 *svals = nullptr;
   int *ranges = nullptr, *nchunks = nullptr;
   int4 *chunks = nullptr;
   void *temp = nullptr;
@@ -122,7 +126,7 @@ struct Ctx {
   std::vector<float> h_w, h_h;
   std::vector<uint8_t> h_kind;
   std::vector<int32_t> h_stage_idx;  // objects of the current stage, in stage order
-  std::vector<int32_t> h_pin_of;     // device pin index of each design pin (nets are reordered)
+  std::vector<int32_t> h_grad_slot;  // pin_grad slot of each design pin (-1: terminal pin)
   float core_llx = 0, core_lly = 0, core_urx = 0, core_ury = 0, total_row_area = 0;
   float bsx = 0, bsy = 0;
   float std_area = 0, macro_area = 0, filler_w = 0, filler_h = 0;
@@ -148,9 +152,9 @@ struct Ctx {
   int32_t *group_begin = nullptr, *group_end = nullptr;  // pin ranges of the 64-pin net groups
   int num_groups = 0;
   int32_t *pin_net = nullptr;  // net of each pin
-  int32_t *node_pin_start = nullptr;  // [N+1]
-  int32_t *node_pins = nullptr;
-  float2 *pin_grad = nullptr;
+  int32_t *node_pin_start = nullptr;  // [N+1] runs of pin_grad per node
+  int32_t *pin_slot = nullptr;        // [P] pin_grad slot of each device pin, -1 for terminal pins
+  float2 *pin_grad = nullptr;         // [node pins] WA gradients, node by node in modulePins order
   // bins
   float *bllx = nullptr, *burx = nullptr, *blly = nullptr, *bury = nullptr;
   unsigned long long *node_q = nullptr, *filler_q = nullptr;

@@ -41,7 +41,7 @@ __global__ void copy_float2_kernel(int64_t n, float2 *dst, const float2 *src) {
 // (nesterov.hpp:375-415).  Iteration 0 uses step 1.  A non-finite 1/L
 // (identical consecutive positions, 0/0) keeps the previous step and is
 // counted in bad_numeric.
-__global__ void __launch_bounds__(kBlock) step_scalar_kernel(int nblocks, double *partials, DevScalars *sc) {
+__global__ void __launch_bounds__(kFinish) step_scalar_kernel(int nblocks, double *partials, DevScalars *sc) {
   if (sc->done) return;
   double num = block_sum_partials_strided(partials + kPartialLip, nblocks, 2);
   double den = block_sum_partials_strided(partials + kPartialLip + 1, nblocks, 2);
@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(kBlock) step_scalar_kernel(int nblocks, double
 
 // Backtracking acceptance (nesterov.hpp:337-364): accept when
 // 0.95 * step <= 1/L', L' from the trial evaluation; step := 1/L' either way.
-__global__ void __launch_bounds__(kBlock) accept_kernel(int nblocks, double *partials, DevScalars *sc) {
+__global__ void __launch_bounds__(kFinish) accept_kernel(int nblocks, double *partials, DevScalars *sc) {
   if (sc->done) return;
   double num = block_sum_partials_strided(partials + kPartialLip, nblocks, 2);
   double den = block_sum_partials_strided(partials + kPartialLip + 1, nblocks, 2);
@@ -179,12 +179,12 @@ void launch_copy_float2(Ctx &c, float2 *dst, const float2 *src, int64_t n) {
 }
 
 void launch_step_scalar(Ctx &c, int) {
-  hipLaunchKernelGGL(step_scalar_kernel, dim3(1), dim3(kBlock), 0, c.stream, object_blocks(c.O), c.partials,
+  hipLaunchKernelGGL(step_scalar_kernel, dim3(1), dim3(kFinish), 0, c.stream, object_blocks(c.O), c.partials,
                      c.sc);
 }
 
 void launch_accept(Ctx &c) {
-  hipLaunchKernelGGL(accept_kernel, dim3(1), dim3(kBlock), 0, c.stream, object_blocks(c.O), c.partials, c.sc);
+  hipLaunchKernelGGL(accept_kernel, dim3(1), dim3(kFinish), 0, c.stream, object_blocks(c.O), c.partials, c.sc);
 }
 
 void launch_nesterov_move_ex(Ctx &c, const float2 *ref, const float2 *grad, const float2 *main_in,

@@ -35,12 +35,14 @@ __device__ __forceinline__ double block_sum(double v) {
 
 __device__ __forceinline__ double block_sum_partials(const double *__restrict__ p, int n) {
   double s = 0;
+#pragma unroll 4
   for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
   return block_sum(s);
 }
 
 __device__ __forceinline__ double block_sum_partials_strided(const double *p, int n, int stride) {
   double s = 0;
+#pragma unroll 4
   for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[(size_t)i * stride];
   return block_sum(s);
 }

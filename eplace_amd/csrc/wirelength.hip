@@ -21,7 +21,7 @@ namespace ep {
 // Pin positions: centre of the pin's object or terminal (one packed 8-byte
 // read) plus the pin offset (objects.cpp:454-459).
 struct PinSrc {
-  const int32_t *net_start, *pin_obj;
+  const int32_t *net_start, *pin_obj, *slot;
   const float *pdx, *pdy;
   const float2 *pos;
   __device__ __forceinline__ float2 at(int p) const {
@@ -151,7 +151,8 @@ __global__ void __launch_bounds__(kBlock) wl_large_kernel(int num, const int32_t
       pin_terms<MODEL, EXPM>(py, ymax, ymin, igy, fp, fm);
       float gx = pin_grad_axis<MODEL>(px, xmax, xmin, igx, ep, em, a.dpx, a.npx, a.dmx, a.nmx);
       float gy = pin_grad_axis<MODEL>(py, ymax, ymin, igy, fp, fm, a.dpy, a.npy, a.dmy, a.nmy);
-      pin_grad[p] = make_float2(gx, gy);
+      const int sl = src.slot[p];
+      if (sl >= 0) pin_grad[sl] = make_float2(gx, gy);
     }
   }
   hp = block_sum(hp);
@@ -241,7 +242,8 @@ __global__ void __launch_bounds__(kBlock) wl_group_kernel(int num_groups, const 
     if (act) {
       float gx = pin_grad_axis<MODEL>(px, xmax, xmin, igx, ep, em, dpx, npx, dmx, nmx);
       float gy = pin_grad_axis<MODEL>(py, ymax, ymin, igy, fp, fm, dpy, npy, dmy, nmy);
-      pin_grad[p] = make_float2(gx, gy);
+      const int sl = src.slot[p];
+      if (sl >= 0) pin_grad[sl] = make_float2(gx, gy);
     }
   }
   hp = block_sum(hp);
@@ -255,7 +257,7 @@ __global__ void __launch_bounds__(kBlock) wl_group_kernel(int num_groups, const 
 // Sums the per-block HPWL/WL partials in a fixed order; optionally applies
 // updatePenaltyFactor (eplace.cpp:846-883) with FP64 HPWL and lambda
 // (lambda_update 1), times the controller's factor (lambda_update 2).
-__global__ void __launch_bounds__(kBlock) reduce_wl_kernel(int nslots, double *partials, DevScalars *sc,
+__global__ void __launch_bounds__(kFinish) reduce_wl_kernel(int nslots, double *partials, DevScalars *sc,
                                                             int lambda_update) {
   if (sc->done) return;
   double hp = block_sum_partials_strided(partials + kPartialWL, nslots, 2);
@@ -290,7 +292,7 @@ static int g_large_blocks(const Ctx &c) {
 
 template <int MODEL, int EXPM, int GRAD>
 static void launch_wl_t(Ctx &c) {
-  PinSrc src{c.net_start, c.pin_obj, c.pin_dx, c.pin_dy, c.pos};
+  PinSrc src{c.net_start, c.pin_obj, c.pin_slot, c.pin_dx, c.pin_dy, c.pos};
   int bs = g_small_blocks(c);
   hipLaunchKernelGGL((wl_group_kernel<MODEL, EXPM, GRAD>), dim3(bs), dim3(kBlock), 0, c.stream, c.num_groups,
                      c.group_begin, c.group_end, c.pin_net, src, c.sc, c.pin_grad, c.partials);
@@ -316,7 +318,7 @@ void launch_wirelength(Ctx &c, bool gradient) {
 
 void launch_reduce_wirelength(Ctx &c, int lambda_update) {
   int n = g_small_blocks(c) + (c.num_large ? g_large_blocks(c) : 0);
-  hipLaunchKernelGGL(reduce_wl_kernel, dim3(1), dim3(kBlock), 0, c.stream, n, c.partials, c.sc, lambda_update);
+  hipLaunchKernelGGL(reduce_wl_kernel, dim3(1), dim3(kFinish), 0, c.stream, n, c.partials, c.sc, lambda_update);
 }
 
 }  // namespace ep
