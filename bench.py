@@ -201,6 +201,16 @@ def main():
         print(f"[bench] converged: {rc['iterations']} iterations in {tc:.3f}s, overflow {rc['final_overflow']:.4f}, "
               f"HPWL {rc['final_hpwl']:.6g}", file=sys.stderr, flush=True)
         del restart_t0
+        from eplace_amd.legalize import check_legal, legalize_abacus
+        t0 = time.perf_counter()
+        lxy, lst = legalize_abacus(d, gp.read("positions")[: d.num_nodes], mode=0)
+        chk = check_legal(d, lxy)
+        conv["legalization"] = {"seconds": time.perf_counter() - t0, "legal": chk["legal"],
+                                "mean_displacement": lst["total_displacement"] / max(lst["cells"], 1),
+                                "max_displacement": lst["max_displacement"], "check": chk}
+        print(f"[bench] Abacus: {lst['cells']} cells in {conv['legalization']['seconds']:.2f}s, "
+              f"{'legal' if chk['legal'] else 'NOT legal'}, mean displacement "
+              f"{conv['legalization']['mean_displacement']:.3f}", file=sys.stderr, flush=True)
         if args.method == "bktrk" and not args.no_learned:
             from eplace_amd.controller import Controller
             ctrl = Controller.load()

@@ -88,6 +88,21 @@ class EPRunStats(C.Structure):
     ]
 
 
+class EPLegalParams(C.Structure):
+    _fields_ = [("site_step", C.c_double), ("mode", C.c_int32), ("pad", C.c_int32)]
+
+
+class EPLegalStats(C.Structure):
+    _fields_ = [("cells", C.c_int64), ("subrows", C.c_int32), ("pad", C.c_int32),
+                ("total_displacement", C.c_double), ("max_displacement", C.c_double)]
+
+
+class EPLegalCheck(C.Structure):
+    _fields_ = [("cells", C.c_int64), ("overlaps", C.c_int64), ("off_row", C.c_int64), ("off_site", C.c_int64),
+                ("out_of_core", C.c_int64), ("in_blockage", C.c_int64), ("macro_overlaps", C.c_int64),
+                ("overlap_area", C.c_double)]
+
+
 SYMBOLS = {
     "ep_create": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "ep_destroy": ([C.c_void_p], C.c_int),
@@ -109,6 +124,9 @@ SYMBOLS = {
     "ep_read": ([C.c_void_p, C.c_int32, C.c_void_p, C.c_int64], C.c_int),
     "ep_write": ([C.c_void_p, C.c_int32, C.c_void_p, C.c_int64], C.c_int),
     "ep_last_timings": ([C.c_void_p, C.c_void_p], C.c_int),
+    "ep_host_last_error": ([], C.c_char_p),
+    "ep_legalize_abacus": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "ep_check_legal": ([C.c_void_p, C.c_void_p, C.c_double, C.c_void_p], C.c_int),
     "ep_ctrl_create": ([C.c_int32, C.c_void_p, C.c_uint32, C.c_void_p], C.c_int),
     "ep_ctrl_destroy": ([C.c_void_p], C.c_int),
     "ep_ctrl_last_error": ([C.c_void_p], C.c_char_p),

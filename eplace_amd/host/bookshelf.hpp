@@ -30,6 +30,7 @@ struct BookshelfDesign {
   std::vector<int32_t> pin_obj;
   std::vector<float> pin_dx, pin_dy;
   std::vector<double> row_bottom, row_height, row_x0, row_x1;
+  double site_width = 1.0;
 
   ep_design view() const {
     ep_design d{};
@@ -161,6 +162,7 @@ inline BookshelfDesign read_bookshelf(const std::string &aux) {
         d.row_height.back() = atof(t[2].c_str());
       } else if (t[0] == "Sitewidth" && t.size() >= 3) {
         site = atof(t[2].c_str());
+        d.site_width = site;
       } else if (t.size() >= 6 && (t[3] == "Numsites" || t[3] == "NumSites")) {
         double origin = atof(t[2].c_str()), n = atof(t[5].c_str());
         d.row_x0.back() = origin;

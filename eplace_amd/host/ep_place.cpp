@@ -3,7 +3,7 @@
 //
 //   ep_place -aux <design.aux> [-targetDensity 1.0] [-targetOverflow 0.10]
 //            [-loadpl <file.pl>] [-bktrk] [-LSE] [-bins <n>] [-nomGP]
-//            [-outputPath <dir>] [-dumpDesign <file>]
+//            [-internalLegal] [-outputPath <dir>] [-dumpDesign <file>]
 //
 // Reads the Bookshelf design, runs mGP with EplaceNesterovOpt and writes
 // <outputPath>/<name>/<name>-eGP.pl (lower-left corners, as
@@ -11,6 +11,9 @@
 // library does not hold are reported and skipped: QP (the .pl or -loadpl
 // positions are the start, as with -noQP) and, for designs with macros,
 // mLG + FILLERONLY + cGP (as with -nomLG).  Logging goes to stdout.
+// -internalLegal runs the Abacus legaliser on the result (reference
+// semantics, ePlace_main.cpp:248-259), checks it and writes
+// <name>-legal.pl beside the global placement.
 // -dumpDesign writes the parsed arrays (ep_design field order, raw
 // little-endian) to a file and exits before any device work.
 #include <sys/stat.h>
@@ -30,14 +33,15 @@ namespace {
 struct Args {
   std::string aux, loadpl, out = ".", dump;
   float td = 1.0f, tof = 0.10f;
-  bool bktrk = false, lse = false, no_mgp = false;
+  bool bktrk = false, lse = false, no_mgp = false, legal = false;
   int bins = 0;
 };
 
 int usage(const char *prog) {
   fprintf(stderr,
           "usage: %s -aux <design.aux> [-targetDensity d] [-targetOverflow t] [-loadpl f.pl]\n"
-          "          [-bktrk] [-LSE] [-bins n] [-nomGP] [-outputPath dir] [-dumpDesign file]\n",
+          "          [-bktrk] [-LSE] [-bins n] [-nomGP] [-internalLegal] [-outputPath dir]\n"
+          "          [-dumpDesign file]\n",
           prog);
   return 2;
 }
@@ -74,6 +78,8 @@ bool parse(int argc, char **argv, Args &a) {
       a.lse = true;
     } else if (k == "-nomGP") {
       a.no_mgp = true;
+    } else if (k == "-internalLegal") {
+      a.legal = true;
     } else if (k == "-noQP" || k == "-nomLG" || k == "-nocGP" || k == "-noLegal") {
       // these stages are not run here; accepted for command-line compatibility
     } else {
@@ -159,6 +165,28 @@ int main(int argc, char **argv) {
     std::string pl = dir + d.name + "-eGP.pl";
     write_pl(d, xy, pl);
     printf("wrote %s\n", pl.c_str());
+    if (a.legal) {
+      std::vector<float> lxy(xy.size());
+      ep_legal_params lp{};
+      lp.site_step = d.site_width;
+      lp.mode = 0;
+      ep_legal_stats ls{};
+      if (ep_legalize_abacus(&view, xy.data(), &lp, lxy.data(), &ls) != EP_OK)
+        throw std::runtime_error(std::string("legalisation: ") + ep_host_last_error());
+      ep_legal_check chk{};
+      if (ep_check_legal(&view, lxy.data(), d.site_width, &chk) != EP_OK)
+        throw std::runtime_error(std::string("legality check: ") + ep_host_last_error());
+      const bool ok = chk.overlaps == 0 && chk.off_row == 0 && chk.off_site == 0 && chk.out_of_core == 0 &&
+                      chk.in_blockage == 0 && chk.macro_overlaps == 0;
+      printf("legalised %lld cells: total displacement %.1f, max %.2f; %s (overlaps %lld, off-row %lld, "
+             "off-site %lld, in blockage %lld)\n",
+             (long long)ls.cells, ls.total_displacement, ls.max_displacement, ok ? "legal" : "NOT legal",
+             (long long)chk.overlaps, (long long)chk.off_row, (long long)chk.off_site, (long long)chk.in_blockage);
+      std::string lpl = dir + d.name + "-legal.pl";
+      write_pl(d, lxy, lpl);
+      printf("wrote %s\n", lpl.c_str());
+      if (!ok) return 3;
+    }
   } catch (const std::exception &e) {
     fprintf(stderr, "ep_place: %s\n", e.what());
     return 1;

@@ -1,0 +1,50 @@
+"""Standard-cell legalisation (AbacusLegalizer, Legalization/legalizer.cpp:58-680)
+and a legality checker, through the library's host entry points."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .design import Design
+from .placer import design_struct
+
+
+def _centres(d: Design, xy):
+    if xy is None:
+        return np.ascontiguousarray(np.stack([d.node_x, d.node_y], 1), np.float32)
+    xy = np.ascontiguousarray(xy, np.float32).reshape(-1, 2)
+    if len(xy) < d.num_nodes:
+        raise ValueError(f"need {d.num_nodes} centres, got {len(xy)}")
+    return np.ascontiguousarray(xy[: d.num_nodes])
+
+
+def legalize_abacus(d: Design, xy=None, mode: int = 0, site_step: float | None = None):
+    """Legal node centres [N, 2] and stats; mode 0 = reference semantics, 1 = corrected."""
+    lib = L.load()
+    sd, keep = design_struct(d)
+    inp = _centres(d, xy)
+    out = np.zeros_like(inp)
+    p = L.EPLegalParams(site_step=float(site_step or d.site_width), mode=int(mode))
+    st = L.EPLegalStats()
+    rc = lib.ep_legalize_abacus(C.byref(sd), inp.ctypes.data, C.byref(p), out.ctypes.data, C.byref(st))
+    if rc != L.EP_OK:
+        raise L.EPError(rc, lib.ep_host_last_error().decode())
+    del keep
+    return out, {k: getattr(st, k) for k, _ in L.EPLegalStats._fields_ if k != "pad"}
+
+
+def check_legal(d: Design, xy=None, site_step: float | None = None) -> dict:
+    lib = L.load()
+    sd, keep = design_struct(d)
+    inp = _centres(d, xy)
+    r = L.EPLegalCheck()
+    rc = lib.ep_check_legal(C.byref(sd), inp.ctypes.data, float(site_step or d.site_width), C.byref(r))
+    if rc != L.EP_OK:
+        raise L.EPError(rc, lib.ep_host_last_error().decode())
+    del keep
+    out = {k: getattr(r, k) for k, _ in L.EPLegalCheck._fields_}
+    out["legal"] = all(out[k] == 0 for k in ("overlaps", "off_row", "off_site", "out_of_core", "in_blockage",
+                                               "macro_overlaps"))
+    return out

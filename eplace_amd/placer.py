@@ -25,6 +25,25 @@ def _ptr(a):
     return a.ctypes.data if a is not None and a.size else None
 
 
+def design_struct(d: Design):
+    """The ep_design view of a Design; keep the returned arrays alive with it."""
+    a = lambda v, dt: np.ascontiguousarray(v, dtype=dt)
+    arrays = dict(
+        node_w=a(d.node_w, np.float32), node_h=a(d.node_h, np.float32),
+        node_x=a(d.node_x, np.float32), node_y=a(d.node_y, np.float32),
+        term_w=a(d.term_w, np.float32), term_h=a(d.term_h, np.float32),
+        term_x=a(d.term_x, np.float32), term_y=a(d.term_y, np.float32),
+        term_ni=a(d.term_ni, np.uint8), net_start=a(d.net_start, np.int64),
+        pin_obj=a(d.pin_obj, np.int32), pin_dx=a(d.pin_dx, np.float32), pin_dy=a(d.pin_dy, np.float32),
+        row_bottom=a(d.row_bottom, np.float64), row_height=a(d.row_height, np.float64),
+        row_x0=a(d.row_x0, np.float64), row_x1=a(d.row_x1, np.float64),
+    )
+    sd = L.EPDesign(num_nodes=d.num_nodes, num_terminals=d.num_terminals, num_nets=d.num_nets,
+                    num_rows=len(d.row_bottom), num_pins=d.num_pins, common_row_height=d.common_row_height,
+                    **{k: _ptr(v) for k, v in arrays.items()})
+    return sd, arrays
+
+
 class EPlacer2D:
     """GPU global-placement evaluator (EPlacer_2D)."""
 
@@ -33,21 +52,8 @@ class EPlacer2D:
                  poisson_mode: int = 0, device: int = 0):
         self.lib = L.load()
         self.design = design
-        d = design
         a = lambda v, dt: np.ascontiguousarray(v, dtype=dt)
-        self._arrays = dict(
-            node_w=a(d.node_w, np.float32), node_h=a(d.node_h, np.float32),
-            node_x=a(d.node_x, np.float32), node_y=a(d.node_y, np.float32),
-            term_w=a(d.term_w, np.float32), term_h=a(d.term_h, np.float32),
-            term_x=a(d.term_x, np.float32), term_y=a(d.term_y, np.float32),
-            term_ni=a(d.term_ni, np.uint8), net_start=a(d.net_start, np.int64),
-            pin_obj=a(d.pin_obj, np.int32), pin_dx=a(d.pin_dx, np.float32), pin_dy=a(d.pin_dy, np.float32),
-            row_bottom=a(d.row_bottom, np.float64), row_height=a(d.row_height, np.float64),
-            row_x0=a(d.row_x0, np.float64), row_x1=a(d.row_x1, np.float64),
-        )
-        sd = L.EPDesign(num_nodes=d.num_nodes, num_terminals=d.num_terminals, num_nets=d.num_nets,
-                        num_rows=len(d.row_bottom), num_pins=d.num_pins, common_row_height=d.common_row_height,
-                        **{k: _ptr(v) for k, v in self._arrays.items()})
+        sd, self._arrays = design_struct(design)
         bx, by = (bins, bins) if np.isscalar(bins) else bins
         self._filler_xy = None if filler_xy is None else a(filler_xy, np.float32).reshape(-1)
         p = L.EPParams(target_density=float(target_density), bins_x=int(bx), bins_y=int(by),

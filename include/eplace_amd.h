@@ -200,6 +200,45 @@ int ep_write(ep_ctx *ctx, int32_t which, const void *host, int64_t count);
 int ep_last_timings(ep_ctx *ctx, double *seconds5);
 
 /* ---------------------------------------------------------------------------
+ * Standard-cell legalisation (host, no device work).  These take the design
+ * and node centres directly; errors are described by ep_host_last_error().
+ * ------------------------------------------------------------------------ */
+typedef struct ep_legal_params {
+  double site_step;         /* site width of the rows (.scl Sitewidth) */
+  int32_t mode;             /* 0 reference semantics, 1 corrected (see legalize.cpp) */
+  int32_t pad;
+} ep_legal_params;
+
+typedef struct ep_legal_stats {
+  int64_t cells;            /* standard cells placed */
+  int32_t subrows;          /* row intervals after removing blocked sites */
+  int32_t pad;
+  double total_displacement, max_displacement;
+} ep_legal_stats;
+
+typedef struct ep_legal_check {
+  int64_t cells;            /* standard cells checked */
+  int64_t overlaps;         /* cells overlapping the cell before them in their row */
+  int64_t off_row;          /* lower-left y on no row, or outside the row's x span */
+  int64_t off_site;         /* x not on the row's site grid */
+  int64_t out_of_core;
+  int64_t in_blockage;      /* overlapping a macro or a terminal inside the core */
+  int64_t macro_overlaps;   /* overlapping macro pairs */
+  double overlap_area;
+} ep_legal_check;
+
+const char *ep_host_last_error(void);
+/* AbacusLegalizer::legalization (Legalization/legalizer.cpp:58-177) on the
+ * intervals of PlaceDB::removeBlockedSite (placedb.cpp:629-849).  node_xy:
+ * [num_nodes][2] centres (NULL: the design's); out_xy receives the legal
+ * centres (macros unchanged). */
+int ep_legalize_abacus(const ep_design *design, const float *node_xy, const ep_legal_params *params,
+                       float *out_xy, ep_legal_stats *stats);
+/* No counterpart in the reference (it has no standard-cell overlap
+ * checker): row, site, core, blockage and overlap checks of a placement. */
+int ep_check_legal(const ep_design *design, const float *node_xy, double site_step, ep_legal_check *out);
+
+/* ---------------------------------------------------------------------------
  * Learned step / lambda controller.  No counterpart in the reference: an MLP
  * EP_CTRL_FEATURES -> EP_CTRL_HIDDEN -> EP_CTRL_HIDDEN -> EP_CTRL_OUTPUTS
  * (tanh hidden layers, FP32) evaluated on the device once per iteration of

@@ -13,6 +13,9 @@
 //   fft  -nx NX -ny NY -bx BX -by BY -in RHO.f32 -out DIR
 //   time -aux F.aux -targetDensity TD [-noQP] [-loadpl F.pl] [-iters K]
 //        (wall time of K vanilla iterations after initialisation, 1 thread)
+//   legal -aux F.aux [-centers C.f32] -out DIR
+//        (AbacusLegalizer on the node centres in C.f32, float pairs in
+//        dbNodes order set through setModuleCenter_2D; default: the .pl)
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -367,6 +370,31 @@ int mode_time(int argc, char **argv) {
   return 0;
 }
 
+int mode_legal(int argc, char **argv) {
+  BookshelfParser parser;
+  PlaceDB *db = new PlaceDB();
+  load_design(argc, argv, parser, db);
+  dump_design(db);
+  std::string cin = arg_str(argc, argv, "centers", "");
+  if (!cin.empty()) {
+    std::vector<float> xy(db->dbNodes.size() * 2);
+    FILE *f = fopen(cin.c_str(), "rb");
+    if (!f || fread(xy.data(), 4, xy.size(), f) != xy.size()) {
+      fprintf(stderr, "bad centers file\n");
+      return 2;
+    }
+    fclose(f);
+    for (size_t i = 0; i < db->dbNodes.size(); ++i) db->setModuleCenter_2D(db->dbNodes[i], xy[2 * i], xy[2 * i + 1]);
+  }
+  npy_f32("input_positions", positions(db->dbNodes), 2);
+  scalar("input_hpwl", db->calcHPWL());
+  AbacusLegalizer legal(db);
+  legal.legalization();
+  npy_f32("legal_positions", positions(db->dbNodes), 2);
+  scalar("legal_hpwl", db->calcHPWL());
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -384,6 +412,7 @@ int main(int argc, char **argv) {
   if (mode == "gp") return mode_gp(argc, argv);
   if (mode == "fft") return mode_fft(argc, argv);
   if (mode == "time") return mode_time(argc, argv);
+  if (mode == "legal") return mode_legal(argc, argv);
   fprintf(stderr, "unknown mode %s\n", mode.c_str());
   return 2;
 }

@@ -1,7 +1,7 @@
 """Generate golden fixtures from the reference placer (run in the container
 that has the reference tree; the fixtures are committed, the reference is not).
 
-    python tests/golden/gen_golden.py
+    python tests/golden/gen_golden.py [gp] [poisson] [legal]   (default: all)
 
 Builds oracle/_ref/ref_harness from the reference's own sources
 (oracle/Makefile.ref), runs it on the reference's smoke design and on seeded
@@ -29,7 +29,8 @@ KEEP = ["nodes_whxy", "nodes_macro", "terms_whxyll", "terms_ni", "pin_obj", "pin
         "fillers_init", "fillers_wh", "init_ll", "init_positions", "init_map_node", "init_map_filler",
         "init_map_terminal", "init_map_base", "init_map_ex", "init_map_ey", "init_map_phi", "init_grad_density",
         "init_grad_wirelength", "init_grad_total", "it0_main", "it1_main", "it2_main", "it0_db_positions",
-        "it1_db_positions", "it2_db_positions", "trajectory", "final_positions", "legal_positions"]
+        "it1_db_positions", "it2_db_positions", "trajectory", "final_positions", "legal_positions",
+        "input_positions"]
 
 
 def run(args, out):
@@ -58,9 +59,19 @@ def pack(name, out, extra=None):
     print(f"{name}: {len(arrays)} arrays, iterations {scal.get('iterations')}, final HPWL {scal.get('final_hpwl')}")
 
 
-def main():
+def main(parts):
     subprocess.run(["make", "-s", "-j8", "-f", "oracle/Makefile.ref", f"REF={REF}"], cwd=ROOT, check=True)
     tmp = tempfile.mkdtemp(prefix="golden_")
+    if "gp" in parts:
+        gp_fixtures(tmp)
+    if "poisson" in parts:
+        poisson_fixtures(tmp)
+    if "legal" in parts:
+        legal_fixtures(tmp)
+    shutil.rmtree(tmp)
+
+
+def gp_fixtures(tmp):
     smoke = os.path.join(REF, "examples", "smoke", "smoke.aux")
     # 1. the reference's own smoke flow (QP -> mGP -> Abacus), default and -bktrk
     run(["gp", "-aux", smoke, "-targetDensity", "0.8", "-targetOverflow", "0.1", "-legal"], tmp + "/s")
@@ -83,6 +94,9 @@ def main():
         run(["gp", "-aux", aux, "-targetDensity", str(td), "-targetOverflow", "0.1", "-noQP",
              "-loadpl", aux[:-4] + ".pl", "-dumpEvery", "1000"], tmp + "/out_" + name)
         pack(name, tmp + "/out_" + name, {"target_density": td})
+
+
+def poisson_fixtures(tmp):
     # 3. the Poisson solve alone on random densities (square and non-square bins)
     rng = np.random.default_rng(5)
     fft = {}
@@ -98,8 +112,28 @@ def main():
             fft[key + "_" + k] = np.load(os.path.join(tmp, "fft", k + ".npy"))
     np.savez_compressed(os.path.join(HERE, "poisson.npz"), **fft)
     print("poisson: ", sorted(fft))
-    shutil.rmtree(tmp)
+
+
+def legal_fixtures(tmp):
+    # 4. AbacusLegalizer on given centres: the 2000-cell design's GP result,
+    # and a 20000-cell design from uniformly random centres
+    legal_cases = {
+        "legal_synth_c2": (synthetic(num_cells=2000, num_nets=2100, seed=12, name="legal_synth_c2"),
+                           np.load(os.path.join(HERE, "synth_c2.npz"))["final_positions"][:2000]),
+    }
+    d = synthetic(num_cells=20000, num_nets=21000, seed=21, util=0.7, name="legal_rand20k")
+    llx, lly, urx, ury = d.core_region()
+    rng = np.random.default_rng(22)
+    cx = rng.uniform(llx + d.node_w / 2, urx - d.node_w / 2).astype(np.float32)
+    cy = rng.uniform(lly + d.node_h / 2, ury - d.node_h / 2).astype(np.float32)
+    legal_cases["legal_rand20k"] = (d, np.stack([cx, cy], 1))
+    for name, (d, xy) in legal_cases.items():
+        aux = write_bookshelf(d, os.path.join(tmp, name))
+        f = os.path.join(tmp, name + ".f32")
+        np.ascontiguousarray(xy, np.float32).tofile(f)
+        run(["legal", "-aux", aux, "-noQP", "-centers", f], tmp + "/out_" + name)
+        pack(name, tmp + "/out_" + name)
 
 
 if __name__ == "__main__":
-    main()
+    main(set(sys.argv[1:]) or {"gp", "poisson", "legal"})

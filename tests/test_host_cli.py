@@ -104,9 +104,10 @@ def test_cli_matches_python_api(tmp_path):
     d = synthetic(num_cells=3000, num_nets=3200, seed=2, name="cli3k")
     aux = write_bookshelf(d, str(tmp_path / "in"))
     print("ep_place on cli3k (3000 cells, bins 64)", flush=True)
-    r = subprocess.run([exe(), "-aux", aux, "-bins", "64", "-outputPath", str(tmp_path / "out")],
+    r = subprocess.run([exe(), "-aux", aux, "-bins", "64", "-internalLegal", "-outputPath", str(tmp_path / "out")],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
+    assert "; legal (" in r.stdout, r.stdout
     hpwl_cli = float([ln for ln in r.stdout.splitlines() if ln.startswith("Final HPWL")][0].split()[-1])
 
     e = read_bookshelf(aux)
