@@ -3,7 +3,6 @@
 // Compiled with -ffp-contract=off: the footprint and the per-(object, bin)
 // contribution reproduce the reference's float/double arithmetic exactly, so
 // the int64 fixed-point maps are bit-identical to the CPU oracle's.
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 
@@ -91,75 +90,106 @@ __device__ __forceinline__ unsigned long long fixed_contribution(float s, double
 // ---------------------------------------------------------------------------
 constexpr int kTile = 32, kTileHalo = 3, kTileWin = kTile + kTileHalo, kTileChunk = 1024;
 
-__global__ void __launch_bounds__(kBlock) tile_key_kernel(int O, const float2 *__restrict__ pos,
-                                                           const float *__restrict__ w,
-                                                           const float *__restrict__ h,
-                                                           const uint8_t *__restrict__ kind,
-                                                           const uint8_t *__restrict__ is_big, Grid g, int tyc,
-                                                           uint64_t *keys, float4 *vals, int *ranges,
-                                                           int nranges) {
-  int o = blockIdx.x * blockDim.x + threadIdx.x;
-  if (o < nranges) ranges[o] = 0;
-  if (o >= O) return;
-  const float2 p = pos[o];
-  float cx = p.x, cy = p.y, ww = w[o], hh = h[o];
-  uint32_t key = kTileKeySentinel;
-  if (!is_big[o]) {
-    Foot f = footprint(cx, cy, ww, hh, g);
-    int ti = min(f.i0, g.nx - 1) / kTile, tj = min(f.j0, g.ny - 1) / kTile;
-    key = (uint32_t)kind[o] * kTileKeyStride + (uint32_t)(ti * tyc + tj);
+constexpr int kSortThreads = 1024;
+
+__device__ __forceinline__ uint32_t tile_key(const float2 p, float w, float h, uint8_t kind, uint8_t big,
+                                             const Grid &g, int tyc, int ntiles) {
+  if (big) return 3u * ntiles;
+  const Foot f = footprint(p.x, p.y, w, h, g);
+  const int ti = min(f.i0, g.nx - 1) / kTile, tj = min(f.j0, g.ny - 1) / kTile;
+  return (uint32_t)kind * ntiles + (uint32_t)(ti * tyc + tj);
+}
+
+// Pass 1: block b counts the keys of objects [b per, (b+1) per) in LDS and
+// reserves its run inside each key with one atomic on the key's running
+// total (bcount[b][k] = the run's offset inside the key).
+__global__ void __launch_bounds__(kSortThreads) tile_hist_kernel(
+    int O, int per, const float2 *__restrict__ pos, const float *__restrict__ w, const float *__restrict__ h,
+    const uint8_t *__restrict__ kind, const uint8_t *__restrict__ is_big, Grid g, int tyc, int ntiles, int nkeys,
+    uint32_t *okey, uint32_t *bcount, uint32_t *ktotal) {
+  extern __shared__ uint32_t hist[];
+  for (int k = threadIdx.x; k < nkeys; k += blockDim.x) hist[k] = 0;
+  __syncthreads();
+  const int o0 = blockIdx.x * per, o1 = min(O, o0 + per);
+  for (int o = o0 + threadIdx.x; o < o1; o += blockDim.x) {
+    const uint32_t k = tile_key(pos[o], w[o], h[o], kind[o], is_big[o], g, tyc, ntiles);
+    okey[o] = k;
+    atomicAdd(&hist[k], 1u);
   }
-  keys[o] = (uint64_t)key << 32 | (uint32_t)o;
-  vals[o] = make_float4(cx, cy, ww, hh);
+  __syncthreads();
+  for (int k = threadIdx.x; k < nkeys; k += blockDim.x) {
+    const uint32_t n = hist[k];
+    bcount[(size_t)blockIdx.x * nkeys + k] = n ? atomicAdd(&ktotal[k], n) : 0u;
+  }
 }
 
-// begin/end of each key's run in the sorted order (ranges[2k], ranges[2k+1]).
-__global__ void tile_range_kernel(int O, const uint64_t *__restrict__ keys, int *ranges) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= O) return;
-  uint32_t k = (uint32_t)(keys[i] >> 32);
-  if (k >= kTileKeySentinel) return;
-  if (i == 0 || (uint32_t)(keys[i - 1] >> 32) != k) ranges[2 * k] = i;
-  if (i == O - 1 || (uint32_t)(keys[i + 1] >> 32) != k) ranges[2 * k + 1] = i + 1;
-}
-
-// One block: chunks of <= kTileChunk objects per key run, as (key, begin, end).
-__global__ void __launch_bounds__(1024) tile_chunk_kernel(int nkeys, const int *__restrict__ ranges,
-                                                           int4 *chunks, int *nchunks) {
-  __shared__ int part[1024];
+// Pass 2 (one block): key totals -> first slot of each key (the totals are
+// cleared for the next sort), and chunks of <= kTileChunk objects per key
+// (the sentinel key makes none).
+__global__ void __launch_bounds__(kSortThreads) tile_keyscan_kernel(int nkeys, uint32_t *ktotal, uint32_t *kstart,
+                                                                     int4 *chunks, int *nchunks) {
+  __shared__ int pslot[kSortThreads], pchunk[kSortThreads];
   const int per = (nkeys + blockDim.x - 1) / blockDim.x;
   const int k0 = threadIdx.x * per, k1 = min(k0 + per, nkeys);
-  int cnt = 0;
+  int slots = 0, cnt = 0;
   for (int k = k0; k < k1; ++k) {
-    int n = ranges[2 * k + 1] - ranges[2 * k];
-    cnt += (n + kTileChunk - 1) / kTileChunk;
+    const int n = (int)ktotal[k];
+    slots += n;
+    if (k < nkeys - 1) cnt += (n + kTileChunk - 1) / kTileChunk;
   }
-  part[threadIdx.x] = cnt;
+  pslot[threadIdx.x] = slots;
+  pchunk[threadIdx.x] = cnt;
   __syncthreads();
-  for (int off = 1; off < (int)blockDim.x; off <<= 1) {  // inclusive scan
-    int v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+  for (int off = 1; off < (int)blockDim.x; off <<= 1) {  // inclusive scans
+    const int a = threadIdx.x >= (unsigned)off ? pslot[threadIdx.x - off] : 0;
+    const int b = threadIdx.x >= (unsigned)off ? pchunk[threadIdx.x - off] : 0;
     __syncthreads();
-    part[threadIdx.x] += v;
+    pslot[threadIdx.x] += a;
+    pchunk[threadIdx.x] += b;
     __syncthreads();
   }
-  int pos = part[threadIdx.x] - cnt;
+  int slot = pslot[threadIdx.x] - slots, pos = pchunk[threadIdx.x] - cnt;
   for (int k = k0; k < k1; ++k) {
-    int b = ranges[2 * k], e = ranges[2 * k + 1];
-    for (int c = b; c < e; c += kTileChunk) chunks[pos++] = make_int4(k, c, min(c + kTileChunk, e), 0);
+    const int n = (int)ktotal[k];
+    ktotal[k] = 0;
+    kstart[k] = (uint32_t)slot;
+    if (k < nkeys - 1)
+      for (int c = 0; c < n; c += kTileChunk) chunks[pos++] = make_int4(k, slot + c, slot + min(c + kTileChunk, n), 0);
+    slot += n;
   }
-  if (threadIdx.x == blockDim.x - 1) *nchunks = part[threadIdx.x];
+  if (threadIdx.x == blockDim.x - 1) *nchunks = pchunk[threadIdx.x];
+}
+
+// Pass 3: block b places its objects at kstart[key] + its offset inside the
+// key + an LDS cursor.
+__global__ void __launch_bounds__(kSortThreads) tile_place_kernel(
+    int O, int per, int nkeys, const uint32_t *__restrict__ okey, const uint32_t *__restrict__ bcount,
+    const uint32_t *__restrict__ kstart, const float2 *__restrict__ pos, const float *__restrict__ w,
+    const float *__restrict__ h, uint32_t *sobj, float4 *svals) {
+  extern __shared__ uint32_t cursor[];
+  for (int k = threadIdx.x; k < nkeys; k += blockDim.x)
+    cursor[k] = kstart[k] + bcount[(size_t)blockIdx.x * nkeys + k];
+  __syncthreads();
+  const int o0 = blockIdx.x * per, o1 = min(O, o0 + per);
+  for (int o = o0 + threadIdx.x; o < o1; o += blockDim.x) {
+    const uint32_t slot = atomicAdd(&cursor[okey[o]], 1u);
+    const float2 p = pos[o];
+    sobj[slot] = (uint32_t)o;
+    svals[slot] = make_float4(p.x, p.y, w[o], h[o]);
+  }
 }
 
 __global__ void __launch_bounds__(kBlock) tile_scatter_kernel(const int4 *__restrict__ chunks,
                                                                const int *__restrict__ nchunks,
                                                                const float4 *__restrict__ vals, Grid g, int tyc,
+                                                               int ntiles,
                                                                double scale, unsigned long long *node_q,
                                                                unsigned long long *filler_q) {
   __shared__ unsigned long long win[kTileWin * kTileWin];
   const int total = *nchunks;
   for (int ci = blockIdx.x; ci < total; ci += gridDim.x) {
     const int4 ch = chunks[ci];
-    const int kind = ch.x / kTileKeyStride, tile = ch.x % kTileKeyStride;
+    const int kind = ch.x / ntiles, tile = ch.x % ntiles;
     const int ib = (tile / tyc) * kTile, jb = (tile % tyc) * kTile;
     unsigned long long *dst = kind == KIND_FILLER ? filler_q : node_q;
     for (int t = threadIdx.x; t < kTileWin * kTileWin; t += blockDim.x) win[t] = 0;
@@ -293,9 +323,9 @@ __device__ __forceinline__ float2 gather_one(const Foot &f, const Grid &g, int l
 // the result is identical.
 __global__ void __launch_bounds__(kBlock) gather_tile_kernel(const int4 *__restrict__ chunks,
                                                               const int *__restrict__ nchunks,
-                                                              const uint64_t *__restrict__ keys,
+                                                              const uint32_t *__restrict__ sobj,
                                                               const float4 *__restrict__ vals, Grid g, int tyc,
-                                                              int ldp, const float *__restrict__ ex,
+                                                              int ntiles, int ldp, const float *__restrict__ ex,
                                                               const float *__restrict__ ey, float2 *gd,
                                                               const DevScalars *sc) {
   if (sc->done) return;
@@ -303,7 +333,7 @@ __global__ void __launch_bounds__(kBlock) gather_tile_kernel(const int4 *__restr
   const int total = *nchunks;
   for (int ci = blockIdx.x; ci < total; ci += gridDim.x) {
     const int4 ch = chunks[ci];
-    const int tile = ch.x % kTileKeyStride;
+    const int tile = ch.x % ntiles;
     const int ib = (tile / tyc) * kTile, jb = (tile % tyc) * kTile;
     __syncthreads();  // the previous chunk is done with the window
     for (int t = threadIdx.x; t < kTileWin * kTileWin; t += blockDim.x) {
@@ -339,7 +369,7 @@ __global__ void __launch_bounds__(kBlock) gather_tile_kernel(const int4 *__restr
           gx += a * e0;
           gy += a * e1;
         }
-      gd[(uint32_t)keys[q]] = make_float2(gx, gy);
+      gd[sobj[q]] = make_float2(gx, gy);
     }
   }
 }
@@ -487,21 +517,15 @@ void launch_scatter(Ctx &c) {
   Grid g = c.grid();
   double scale = ldexp(1.0, c.shift);
   TileSort &t = c.tiles;
-  const int nranges = 2 * kTileKeySentinel;
-  hipLaunchKernelGGL(tile_key_kernel, dim3(blocks_for(std::max(c.O, nranges))), dim3(kBlock), 0, c.stream, c.O,
-                     c.pos, c.w, c.h, c.kind, c.is_big, g, t.tyc, t.keys[0], t.vals[0], t.ranges, nranges);
-  hipcub::DoubleBuffer<uint64_t> kb(t.keys[0], t.keys[1]);
-  hipcub::DoubleBuffer<float4> vb(t.vals[0], t.vals[1]);
-  size_t bytes = t.temp_bytes;
-  (void)hipcub::DeviceRadixSort::SortPairs(t.temp, bytes, kb, vb, c.O, 32, 32 + kTileKeyBits, c.stream);
-  t.skeys = kb.Current();
-  t.svals = vb.Current();
-  hipLaunchKernelGGL(tile_range_kernel, dim3(blocks_for(c.O)), dim3(kBlock), 0, c.stream, c.O, kb.Current(),
-                     t.ranges);
-  hipLaunchKernelGGL(tile_chunk_kernel, dim3(1), dim3(1024), 0, c.stream, kTileKeySentinel, t.ranges, t.chunks,
-                     t.nchunks);
-  hipLaunchKernelGGL(tile_scatter_kernel, dim3(t.grid), dim3(kBlock), 0, c.stream, t.chunks, t.nchunks,
-                     vb.Current(), g, t.tyc, scale, c.node_q, c.filler_q);
+  const size_t lds = (size_t)t.nkeys * sizeof(uint32_t);
+  hipLaunchKernelGGL(tile_hist_kernel, dim3(t.nblocks), dim3(kSortThreads), lds, c.stream, c.O, t.per, c.pos, c.w,
+                     c.h, c.kind, c.is_big, g, t.tyc, t.ntiles, t.nkeys, t.okey, t.bcount, t.ktotal);
+  hipLaunchKernelGGL(tile_keyscan_kernel, dim3(1), dim3(kSortThreads), 0, c.stream, t.nkeys, t.ktotal, t.kstart,
+                     t.chunks, t.nchunks);
+  hipLaunchKernelGGL(tile_place_kernel, dim3(t.nblocks), dim3(kSortThreads), lds, c.stream, c.O, t.per, t.nkeys,
+                     t.okey, t.bcount, t.kstart, c.pos, c.w, c.h, t.sobj, t.svals);
+  hipLaunchKernelGGL(tile_scatter_kernel, dim3(t.grid), dim3(kBlock), 0, c.stream, t.chunks, t.nchunks, t.svals, g,
+                     t.tyc, t.ntiles, scale, c.node_q, c.filler_q);
   if (c.num_big)
     hipLaunchKernelGGL(scatter_big_kernel, dim3(c.num_big), dim3(kBlock), 0, c.stream, c.big, c.pos, c.w,
                        c.h, c.kind, g, scale, c.node_q, c.filler_q);
@@ -512,34 +536,32 @@ int poisson_ld(const Ctx &c);
 bool tile_sort_create(Ctx &c, std::string &err) {
   TileSort &t = c.tiles;
   t.tyc = (c.ny + kTile - 1) / kTile;
-  int txc = (c.nx + kTile - 1) / kTile;
-  if (txc * t.tyc > kTileKeyStride) {
-    err = "grid too large for the tile scatter";
+  t.ntiles = ((c.nx + kTile - 1) / kTile) * t.tyc;
+  t.nkeys = 3 * t.ntiles + 1;
+  if ((size_t)t.nkeys * sizeof(uint32_t) > 64 * 1024) {
+    err = "grid too large for the tile sort";
     return false;
   }
-  size_t n = std::max(c.O, 1);
-  bool ok = hipMalloc(&t.keys[0], n * 8) == hipSuccess && hipMalloc(&t.keys[1], n * 8) == hipSuccess &&
-            hipMalloc(&t.vals[0], n * 16) == hipSuccess && hipMalloc(&t.vals[1], n * 16) == hipSuccess &&
-            hipMalloc(&t.ranges, 2 * kTileKeySentinel * sizeof(int)) == hipSuccess &&
+  const int O = std::max(c.O, 1);
+  t.nblocks = std::max(1, std::min(256, (O + kSortThreads - 1) / kSortThreads));
+  t.per = (O + t.nblocks - 1) / t.nblocks;
+  const size_t max_chunks = (size_t)O / kTileChunk + 3 * (size_t)t.ntiles + 1;
+  bool ok = hipMalloc(&t.okey, (size_t)O * 4) == hipSuccess &&
+            hipMalloc(&t.bcount, (size_t)t.nkeys * t.nblocks * 4) == hipSuccess &&
+            hipMalloc(&t.ktotal, (size_t)t.nkeys * 4) == hipSuccess &&
+            hipMalloc(&t.kstart, (size_t)t.nkeys * 4) == hipSuccess &&
+            hipMalloc(&t.sobj, (size_t)O * 4) == hipSuccess && hipMalloc(&t.svals, (size_t)O * 16) == hipSuccess &&
+            hipMalloc(&t.chunks, max_chunks * sizeof(int4)) == hipSuccess &&
             hipMalloc(&t.nchunks, sizeof(int)) == hipSuccess;
-  size_t max_chunks = n / kTileChunk + 3 * (size_t)kTileKeyStride + 1;
-  ok = ok && hipMalloc(&t.chunks, max_chunks * sizeof(int4)) == hipSuccess;
-  if (ok) {
-    hipcub::DoubleBuffer<uint64_t> kb(t.keys[0], t.keys[1]);
-    hipcub::DoubleBuffer<float4> vb(t.vals[0], t.vals[1]);
-    t.temp_bytes = 0;
-    ok = hipcub::DeviceRadixSort::SortPairs(nullptr, t.temp_bytes, kb, vb, (int)n, 32, 32 + kTileKeyBits) ==
-             hipSuccess &&
-         hipMalloc(&t.temp, std::max<size_t>(t.temp_bytes, 16)) == hipSuccess;
-  }
+  ok = ok && hipMemset(t.ktotal, 0, (size_t)t.nkeys * 4) == hipSuccess;
   t.grid = (int)std::min<size_t>(max_chunks, 2048);
-  if (!ok) err = "tile scatter allocation failed";
+  if (!ok) err = "tile sort allocation failed";
   return ok;
 }
 
 void tile_sort_destroy(Ctx &c) {
   TileSort &t = c.tiles;
-  void *p[] = {t.keys[0], t.keys[1], t.vals[0], t.vals[1], t.ranges, t.nchunks, t.chunks, t.temp};
+  void *p[] = {t.okey, t.bcount, t.ktotal, t.kstart, t.sobj, t.svals, t.chunks, t.nchunks};
   for (void *q : p)
     if (q) (void)hipFree(q);
   t = TileSort();
@@ -562,8 +584,8 @@ void launch_gather_combine(Ctx &c, bool recompute, bool lipschitz, const float2 
   int ldp = poisson_ld(c);
   if (recompute) {
     TileSort &t = c.tiles;
-    hipLaunchKernelGGL(gather_tile_kernel, dim3(t.grid), dim3(kBlock), 0, c.stream, t.chunks, t.nchunks, t.skeys,
-                       t.svals, g, t.tyc, ldp, c.ex, c.ey, c.gd, c.sc);
+    hipLaunchKernelGGL(gather_tile_kernel, dim3(t.grid), dim3(kBlock), 0, c.stream, t.chunks, t.nchunks, t.sobj,
+                       t.svals, g, t.tyc, t.ntiles, ldp, c.ex, c.ey, c.gd, c.sc);
     if (c.num_big)
       hipLaunchKernelGGL(gather_big_kernel, dim3(c.num_big), dim3(kBlock), 0, c.stream, c.big, c.pos, c.w, c.h,
                          g, ldp, c.ex, c.ey, c.gd, c.sc);

@@ -83,22 +83,21 @@ constexpr int kPartialTotal = 10 * kMaxPartials;
 
 struct Poisson;  // poisson.hip
 
-// Tile-sorted scatter / gather state (density.hip).  Keys are kind *
-// kTileKeyStride + tile in the high 32 bits and the object in the low 32;
-// kTileKeySentinel marks objects handled by the block-per-object path.
-constexpr uint32_t kTileKeyStride = 4096;
-constexpr uint32_t kTileKeySentinel = 3 * kTileKeyStride;
-constexpr int kTileKeyBits = 14;
+// Counting sort of the objects by (kind, 32x32-bin tile of the footprint's
+// lower-left bin), shared by the scatter and the gather of one evaluation
+// (density.hip).  key = kind * ntiles + tile; key 3 * ntiles marks objects
+// handled by the block-per-object kernels.  Order inside a key is not
+// fixed; every consumer's result is independent of it.
 struct TileSort {
-  int tyc = 0, grid = 0;
-  uint64_t *keys[2] = {nullptr, nullptr};  // key << 32 | object
-  float4 *vals[2] = {nullptr, nullptr};    // centre x, y, width, height
-  const uint64_t *skeys = nullptr;         // sorted by the last scatter
-  const float4 *svals = nullptr;
-  int *ranges = nullptr, *nchunks = nullptr;
-  int4 *chunks = nullptr;
-  void *temp = nullptr;
-  size_t temp_bytes = 0;
+  int tyc = 0, ntiles = 0, nkeys = 0, nblocks = 0, per = 0, grid = 0;
+  uint32_t *okey = nullptr;    // [O] key of each object
+  uint32_t *bcount = nullptr;  // [nblocks][nkeys] offset of each block's run inside the key
+  uint32_t *ktotal = nullptr;  // [nkeys] running key totals, zero between sorts
+  uint32_t *kstart = nullptr;  // [nkeys] first sorted slot of each key
+  uint32_t *sobj = nullptr;    // [O] objects in key order
+  float4 *svals = nullptr;     // [O] their (x, y, w, h)
+  int4 *chunks = nullptr;      // (key, begin, end) runs of <= kTileChunk objects of one key
+  int *nchunks = nullptr;
 };
 
 struct Ctx {
