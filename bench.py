@@ -111,6 +111,42 @@ def cpu_baseline(design, method, iters, init_positions):
             "seconds": dt}
 
 
+def qp_reference_baseline(design, outer_iterations):
+    """The reference's own QPPlacer (oracle/_ref/ref_harness, built from the
+    reference's sources with its vendored Eigen, 8 OpenMP threads as
+    qplace.cpp:50 asks) on the same design for `outer_iterations` outer
+    iterations from the core centre; its per-iteration wall time is the
+    baseline.  None when the harness is not built."""
+    import subprocess
+    import tempfile
+    from eplace_amd.design import write_bookshelf
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if not os.path.exists(exe):
+        return None
+    tmp = tempfile.mkdtemp(prefix="qpref_")
+    try:
+        aux = write_bookshelf(design, os.path.join(tmp, "in"))
+        out = os.path.join(tmp, "out")
+        env = dict(os.environ, OMP_NUM_THREADS="8")
+        r = subprocess.run([exe, "qp", "-noQP", "-timeOnly", "-aux", aux, "-IPiteCount",
+                            str(outer_iterations - 1), "-out", out], capture_output=True, text=True, cwd=tmp,
+                           env=env, timeout=600)
+        if r.returncode != 0:
+            return None
+        per = [float(l.split("CPUtime")[1]) for l in r.stdout.splitlines() if "at iteration number" in l]
+        hp = [float(l.split("HPWL")[1].split(",")[0]) for l in r.stdout.splitlines() if "at iteration number" in l]
+        if not per:
+            return None
+        return {"value": float(np.mean(per)), "unit": "s per outer iteration", "cores": 8, "kind": "reference",
+                "sample": f"{len(per)} outer iteration(s) of the reference QPPlacer::quadraticPlacement "
+                          f"(qplace.cpp, Eigen BiCGSTAB, setNbThreads(8)) on the same design from the core "
+                          f"centre, wall time per iteration as the reference prints it, on {cpu_model()}",
+                "hpwl_log": hp}
+    finally:
+        import shutil
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +159,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-controller convergence run")
     ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--no-qp", action="store_true", help="skip the quadratic-placement run")
+    ap.add_argument("--no-qp-cpu", action="store_true", help="skip the reference QP CPU baseline")
     ap.add_argument("--poisson-mode", type=int, default=1, choices=[0, 1],
                     help="transform arithmetic: 0 FP32 MFMA, 1 split-BF16 MFMA")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -230,6 +268,32 @@ def main():
                   f"overflow {rl['final_overflow']:.4f}, HPWL {rl['final_hpwl']:.6g} "
                   f"({100 * (rl['final_hpwl'] / rc['final_hpwl'] - 1):+.3f}%)", file=sys.stderr, flush=True)
 
+    qp = None
+    if not args.no_qp:
+        from eplace_amd.qplace import quadratic_placement
+        dq = synthetic(cells, nets, seed=args.seed, num_io=1000, util=0.7, name=f"synth_{cells}")
+        quadratic_placement(dq, max_iterations=0)  # warm: code objects, allocator
+        dq = synthetic(cells, nets, seed=args.seed, num_io=1000, util=0.7, name=f"synth_{cells}")
+        t0 = time.perf_counter()
+        sq = quadratic_placement(dq)
+        tq = time.perf_counter() - t0
+        qp = {"workload": "the same design, QPPlacer::quadraticPlacement defaults (from the core centre, "
+                          "-IPiteCount 20, BiCGSTAB 100 iterations)",
+              "seconds": tq, "iterations": sq["iterations"], "solver_iterations": sq["solver_iterations"],
+              "seconds_per_outer_iteration": tq / sq["iterations"], "initial_hpwl": sq["initial_hpwl"],
+              "final_hpwl": sq["final_hpwl"], "error_x": sq["error_x"], "error_y": sq["error_y"],
+              "hpwl_log": sq["log"][:, 0].tolist()}
+        print(f"[bench] QP: {sq['iterations']} outer iterations ({sq['solver_iterations']} BiCGSTAB) in {tq:.3f}s, "
+              f"HPWL {sq['initial_hpwl']:.6g} -> {sq['final_hpwl']:.6g}", file=sys.stderr, flush=True)
+        if not args.no_qp_cpu and not args.no_cpu:
+            base = qp_reference_baseline(dq, 1)
+            if base is not None:
+                qp["cpu_baseline"] = base
+                qp["speedup_per_outer_iteration"] = base["value"] / qp["seconds_per_outer_iteration"]
+                print(f"[bench] QP reference: {base['value']:.2f}s per outer iteration "
+                      f"({qp['speedup_per_outer_iteration']:.0f}x)", file=sys.stderr, flush=True)
+        del dq
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(d, args.method, args.cpu_iters, init_pos)
@@ -257,6 +321,7 @@ def main():
         "roofline": roof,
         "phase_ms_per_evaluation": {k: v * 1e3 for k, v in ph.items()},
         "converge": conv,
+        "qp": qp,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -6,6 +6,7 @@ EPlacer_2D / EplaceNesterovOpt call surface plus design I/O.
 """
 from .design import Design, read_bookshelf, synthetic, write_bookshelf, write_pl  # noqa: F401
 from .placer import CGP, FILLERONLY, MGP, EPlacer2D, EplaceNesterovOpt  # noqa: F401
+from .qplace import quadratic_placement  # noqa: F401
 
-__all__ = ["Design", "read_bookshelf", "synthetic", "write_bookshelf", "write_pl", "EPlacer2D",
+__all__ = ["quadratic_placement", "Design", "read_bookshelf", "synthetic", "write_bookshelf", "write_pl", "EPlacer2D",
            "EplaceNesterovOpt", "MGP", "FILLERONLY", "CGP"]

@@ -103,6 +103,27 @@ class EPLegalCheck(C.Structure):
                 ("overlap_area", C.c_double)]
 
 
+class EPQpParams(C.Structure):
+    _fields_ = [("max_iterations", C.c_int32), ("solver_iterations", C.c_int32), ("target_error", C.c_float),
+                ("solver_tolerance", C.c_float), ("start_at_center", C.c_int32), ("device", C.c_int32)]
+
+
+class EPQpStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("solver_iterations", C.c_int32), ("restarts", C.c_int32),
+                ("pad", C.c_int32), ("initial_hpwl", C.c_double), ("final_hpwl", C.c_double),
+                ("error_x", C.c_double), ("error_y", C.c_double), ("seconds", C.c_double)]
+
+
+class EPQpMatrixOut(C.Structure):
+    _fields_ = [("nnz", C.c_int64), ("capacity", C.c_int64), ("rowptr", C.c_void_p), ("col", C.c_void_p),
+                ("val", C.c_void_p), ("diag", C.c_void_p), ("b", C.c_void_p), ("hpwl", C.c_double)]
+
+
+class EPQpSystem(C.Structure):
+    _fields_ = [("n", C.c_int32), ("ndim", C.c_int32), ("nnz", C.c_int64), ("rowptr", C.c_void_p),
+                ("col", C.c_void_p), ("val", C.c_void_p), ("diag", C.c_void_p), ("b", C.c_void_p)]
+
+
 SYMBOLS = {
     "ep_create": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "ep_destroy": ([C.c_void_p], C.c_int),
@@ -127,6 +148,10 @@ SYMBOLS = {
     "ep_host_last_error": ([], C.c_char_p),
     "ep_legalize_abacus": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "ep_check_legal": ([C.c_void_p, C.c_void_p, C.c_double, C.c_void_p], C.c_int),
+    "ep_qp_last_error": ([], C.c_char_p),
+    "ep_qplace": ([C.c_void_p] * 5, C.c_int),
+    "ep_qp_matrix": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p], C.c_int),
+    "ep_qp_solve": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_float, C.c_int32, C.c_void_p, C.c_void_p], C.c_int),
     "ep_ctrl_create": ([C.c_int32, C.c_void_p, C.c_uint32, C.c_void_p], C.c_int),
     "ep_ctrl_destroy": ([C.c_void_p], C.c_int),
     "ep_ctrl_last_error": ([C.c_void_p], C.c_char_p),

@@ -2,15 +2,17 @@
 // (main/ePlace_main.cpp:35-223) over libeplace_amd.so.
 //
 //   ep_place -aux <design.aux> [-targetDensity 1.0] [-targetOverflow 0.10]
-//            [-loadpl <file.pl>] [-bktrk] [-LSE] [-bins <n>] [-nomGP]
-//            [-internalLegal] [-outputPath <dir>] [-dumpDesign <file>]
+//            [-loadpl <file.pl>] [-noQP] [-IPiteCount <n>] [-bktrk] [-LSE]
+//            [-bins <n>] [-nomGP] [-internalLegal] [-outputPath <dir>]
+//            [-dumpDesign <file>]
 //
-// Reads the Bookshelf design, runs mGP with EplaceNesterovOpt and writes
+// Reads the Bookshelf design, runs QP (QPPlacer::quadraticPlacement from the
+// core centre, ePlace_main.cpp:121-126; -noQP keeps the .pl or -loadpl
+// positions), mGP with EplaceNesterovOpt and writes
 // <outputPath>/<name>/<name>-eGP.pl (lower-left corners, as
-// PlaceDB::outputBookShelf does, placedb.cpp:1022-1060).  The stages the
-// library does not hold are reported and skipped: QP (the .pl or -loadpl
-// positions are the start, as with -noQP) and, for designs with macros,
-// mLG + FILLERONLY + cGP (as with -nomLG).  Logging goes to stdout.
+// PlaceDB::outputBookShelf does, placedb.cpp:1022-1060).  For designs with
+// macros, mLG + FILLERONLY + cGP are reported and skipped (as with -nomLG).
+// Logging goes to stdout.
 // -internalLegal runs the Abacus legaliser on the result (reference
 // semantics, ePlace_main.cpp:248-259), checks it and writes
 // <name>-legal.pl beside the global placement.
@@ -18,6 +20,7 @@
 // little-endian) to a file and exits before any device work.
 #include <sys/stat.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -33,15 +36,15 @@ namespace {
 struct Args {
   std::string aux, loadpl, out = ".", dump;
   float td = 1.0f, tof = 0.10f;
-  bool bktrk = false, lse = false, no_mgp = false, legal = false;
-  int bins = 0;
+  bool bktrk = false, lse = false, no_mgp = false, legal = false, no_qp = false;
+  int bins = 0, qp_iters = 20;
 };
 
 int usage(const char *prog) {
   fprintf(stderr,
           "usage: %s -aux <design.aux> [-targetDensity d] [-targetOverflow t] [-loadpl f.pl]\n"
-          "          [-bktrk] [-LSE] [-bins n] [-nomGP] [-internalLegal] [-outputPath dir]\n"
-          "          [-dumpDesign file]\n",
+          "          [-noQP] [-IPiteCount n] [-bktrk] [-LSE] [-bins n] [-nomGP] [-internalLegal]\n"
+          "          [-outputPath dir] [-dumpDesign file]\n",
           prog);
   return 2;
 }
@@ -80,7 +83,12 @@ bool parse(int argc, char **argv, Args &a) {
       a.no_mgp = true;
     } else if (k == "-internalLegal") {
       a.legal = true;
-    } else if (k == "-noQP" || k == "-nomLG" || k == "-nocGP" || k == "-noLegal") {
+    } else if (k == "-IPiteCount") {
+      if (!val(v)) return false;
+      a.qp_iters = atoi(v.c_str());
+    } else if (k == "-noQP") {
+      a.no_qp = true;
+    } else if (k == "-nomLG" || k == "-nocGP" || k == "-noLegal") {
       // these stages are not run here; accepted for command-line compatibility
     } else {
       fprintf(stderr, "unknown option %s\n", k.c_str());
@@ -131,9 +139,23 @@ int main(int argc, char **argv) {
       dump_design(d, a.dump);
       return 0;
     }
-    printf("QP is not run: starting from the .pl positions\n");
-
     ep_design view = d.view();
+    if (!a.no_qp) {
+      // QPPlacer::quadraticPlacement (qplace.cpp:20-103)
+      std::vector<float> xy(2 * d.node_x.size());
+      for (size_t i = 0; i < d.node_x.size(); ++i) xy[2 * i] = d.node_x[i], xy[2 * i + 1] = d.node_y[i];
+      std::vector<double> log;
+      ep_qp_stats qs = QPPlacer(view).quadraticPlacement(xy, a.qp_iters, &log);
+      printf("INFO:  initial HPWL: %.6lf\n", qs.initial_hpwl);
+      printf("INFO:  The matrix size is %zu\n", d.node_x.size());
+      for (int i = 0; i < qs.iterations; ++i)
+        printf("INFO:  at iteration number %3d,  CG Error %.6lf,  HPWL %.6lf\n", i,
+               std::max(log[4 * i + 1], log[4 * i + 2]), log[4 * i]);
+      printf("QP time: %.4f s\n", qs.seconds);
+      for (size_t i = 0; i < d.node_x.size(); ++i) d.node_x[i] = xy[2 * i], d.node_y[i] = xy[2 * i + 1];
+    } else {
+      printf("QP skipped (-noQP): starting from the .pl positions\n");
+    }
     PlacerOptions o;
     o.bins_x = o.bins_y = a.bins;
     o.lse = a.lse;

@@ -131,6 +131,37 @@ class EPlacer_2D {
   float targetDensity_ = 1.f;
 };
 
+// QPPlacer (QPlace/qplace.h:24-44).  quadraticPlacement() runs the
+// reference's quadratic placement on the device and moves the movable nodes
+// (node_xy: [N][2] centres) to its result, as the reference moves the
+// modules of its database.  log receives one row per outer iteration: HPWL,
+// X error, Y error, BiCGSTAB iterations (the line qplace.cpp:89 prints).
+class QPPlacer {
+ public:
+  explicit QPPlacer(const ep_design &design, int device = 0) : design_(design), device_(device) {}
+  ep_qp_stats quadraticPlacement(std::vector<float> &node_xy, int maxIterations = 20,
+                                 std::vector<double> *log = nullptr) {
+    if (node_xy.size() != 2 * (size_t)design_.num_nodes) throw std::runtime_error("QPPlacer: node_xy size");
+    ep_qp_params p{};
+    p.max_iterations = maxIterations;   // -IPiteCount (qplace.cpp:35-42)
+    p.solver_iterations = 100;          // qplace.cpp:68
+    p.target_error = 0.000001f;         // qplace.cpp:45
+    p.solver_tolerance = 1.1920928955078125e-07f;  // Eigen's default for float
+    p.start_at_center = 1;              // moveNodesCenterToCenter (qplace.cpp:23)
+    p.device = device_;
+    std::vector<double> rows(4 * (size_t)(maxIterations + 1));
+    ep_qp_stats st{};
+    if (ep_qplace(&design_, &p, node_xy.data(), rows.data(), &st) != EP_OK)
+      throw std::runtime_error(std::string("ep_qplace: ") + ep_qp_last_error());
+    if (log) log->assign(rows.begin(), rows.begin() + 4 * (size_t)st.iterations);
+    return st;
+  }
+
+ private:
+  ep_design design_;
+  int device_;
+};
+
 template <typename T>
 class FirstOrderOptimizer {
  public:

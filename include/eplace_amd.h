@@ -228,6 +228,74 @@ typedef struct ep_legal_check {
 } ep_legal_check;
 
 const char *ep_host_last_error(void);
+/* ---------------------------------------------------------------------------
+ * Quadratic initial placement (QPlace/qplace.cpp).  Self-contained: each call
+ * uploads the design, runs on the device and returns host results; errors
+ * are described by ep_qp_last_error().
+ * ------------------------------------------------------------------------- */
+typedef struct ep_qp_params {
+  int32_t max_iterations;    /* -IPiteCount, default 20 (qplace.cpp:35-42) */
+  int32_t solver_iterations; /* BiCGSTAB iteration cap, 100 (qplace.cpp:68) */
+  float target_error;        /* outer stop on both solver errors, 1e-6 (qplace.cpp:45) */
+  float solver_tolerance;    /* BiCGSTAB tolerance: Eigen's default, float epsilon */
+  int32_t start_at_center;   /* 1: moveNodesCenterToCenter first (qplace.cpp:23), as the reference */
+  int32_t device;            /* HIP device ordinal */
+} ep_qp_params;
+
+typedef struct ep_qp_stats {
+  int32_t iterations;        /* outer iterations run */
+  int32_t solver_iterations; /* BiCGSTAB iterations, both directions, all outer iterations */
+  int32_t restarts;          /* BiCGSTAB restarts */
+  int32_t pad;
+  double initial_hpwl;       /* after the move to the centre (qplace.cpp:25) */
+  double final_hpwl;         /* calcNetBoundPins after the last update (qplace.cpp:81) */
+  double error_x, error_y;   /* last solver errors (qplace.cpp:73, 78) */
+  double seconds;            /* wall time of the call, upload included */
+} ep_qp_stats;
+
+/* Assembled systems of QPPlacer::createSparseMatrix: rows 0..N-1 are X and
+ * N..2N-1 are Y; off-diagonal entries in CSR (columns are node indices of
+ * the same direction, duplicates not merged), the diagonal and b apart.
+ * Call with rowptr/col/val/diag/b NULL to get nnz, then with arrays of
+ * [2N+1], [capacity], [capacity], [2N], [2N]. */
+typedef struct ep_qp_matrix_out {
+  int64_t nnz;      /* out */
+  int64_t capacity; /* in: length of col / val */
+  int32_t *rowptr, *col;
+  float *val, *diag, *b;
+  double hpwl;      /* out: calcNetBoundPins at the given centres */
+} ep_qp_matrix_out;
+
+/* ndim systems of n rows in one CSR (rows d*n .. d*n+n-1 belong to system
+ * d, columns 0..n-1), an optional separate diagonal, and b. */
+typedef struct ep_qp_system {
+  int32_t n, ndim;
+  int64_t nnz;
+  const int32_t *rowptr; /* [ndim*n + 1] */
+  const int32_t *col;    /* [nnz] */
+  const float *val;      /* [nnz] */
+  const float *diag;     /* [ndim*n] or NULL */
+  const float *b;        /* [ndim*n] */
+} ep_qp_system;
+
+const char *ep_qp_last_error(void);
+/* QPPlacer::quadraticPlacement (qplace.cpp:20-103).  node_xy: [num_nodes][2]
+ * centres, read when start_at_center is 0 and replaced by the result (the
+ * reference moves the modules of its database).  log: NULL or
+ * [max_iterations + 1][4] doubles: HPWL, X error, Y error, BiCGSTAB
+ * iterations of each outer iteration (the reference's per-iteration line,
+ * qplace.cpp:89). */
+int ep_qplace(const ep_design *design, const ep_qp_params *params, float *node_xy, double *log,
+              ep_qp_stats *stats);
+/* QPPlacer::createSparseMatrix (qplace.cpp:123-280) after calcNetBoundPins,
+ * at the given centres. */
+int ep_qp_matrix(const ep_design *design, const float *node_xy, int32_t device, ep_qp_matrix_out *out);
+/* BiCGSTAB<SMatrix, IdentityPreconditioner>::solveWithGuess with
+ * setMaxIterations (qplace.cpp:67-78) on each of ndim systems; x holds the
+ * guess and receives the solution; error / iterations: [ndim] or NULL. */
+int ep_qp_solve(const ep_qp_system *system, float *x, int32_t max_iterations, float tolerance, int32_t device,
+                float *error, int32_t *iterations);
+
 /* AbacusLegalizer::legalization (Legalization/legalizer.cpp:58-177) on the
  * intervals of PlaceDB::removeBlockedSite (placedb.cpp:629-849).  node_xy:
  * [num_nodes][2] centres (NULL: the design's); out_xy receives the legal

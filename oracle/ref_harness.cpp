@@ -16,6 +16,12 @@
 //   legal -aux F.aux [-centers C.f32] -out DIR
 //        (AbacusLegalizer on the node centres in C.f32, float pairs in
 //        dbNodes order set through setModuleCenter_2D; default: the .pl)
+//   qp   -aux F.aux -noQP [-loadpl F.pl] [-center | -centers C.f32] [-IPiteCount K] [-timeOnly] -out DIR
+//        (QPPlacer::createSparseMatrix at the loaded positions, or at the
+//        core centre with -center, dumped as summed COO plus b and the
+//        guess; then QPPlacer::quadraticPlacement, whose per-iteration
+//        HPWL and solver error go to stdout, and the node centres after it;
+//        -timeOnly: only the wall time and HPWL of quadraticPlacement)
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -395,6 +401,70 @@ int mode_legal(int argc, char **argv) {
   return 0;
 }
 
+void dump_coo(const std::string &tag, const SMatrix &A) {
+  std::vector<int32_t> row, col;
+  std::vector<float> val;
+  for (int k = 0; k < A.outerSize(); ++k)
+    for (SMatrix::InnerIterator it(A, k); it; ++it) {
+      row.push_back((int32_t)it.row());
+      col.push_back((int32_t)it.col());
+      val.push_back(it.value());
+    }
+  npy_i32(tag + "_row", row);
+  npy_i32(tag + "_col", col);
+  npy_f32(tag + "_val", val);
+}
+
+std::vector<float> vecf(const VectorXf &v) { return std::vector<float>(v.data(), v.data() + v.size()); }
+
+int mode_qp(int argc, char **argv) {
+  BookshelfParser parser;
+  PlaceDB *db = new PlaceDB();
+  load_design(argc, argv, parser, db);
+  if (arg_flag(argc, argv, "timeOnly")) {
+    // wall time of quadraticPlacement alone (the reference's own per-
+    // iteration lines on stdout carry the per-iteration times)
+    QPPlacer qp(db);
+    double t0 = now();
+    qp.quadraticPlacement();
+    scalar("qp_seconds", now() - t0);
+    scalar("qp_hpwl", db->calcHPWL());
+    return 0;
+  }
+  dump_design(db);
+  if (arg_flag(argc, argv, "center")) db->moveNodesCenterToCenter();
+  std::string cin = arg_str(argc, argv, "centers", "");
+  if (!cin.empty()) {
+    std::vector<float> xy(db->dbNodes.size() * 2);
+    FILE *f = fopen(cin.c_str(), "rb");
+    if (!f || fread(xy.data(), 4, xy.size(), f) != xy.size()) {
+      fprintf(stderr, "bad centers file\n");
+      return 2;
+    }
+    fclose(f);
+    for (size_t i = 0; i < db->dbNodes.size(); ++i) db->setModuleCenter_2D(db->dbNodes[i], xy[2 * i], xy[2 * i + 1]);
+  }
+  npy_f32("qp_matrix_positions", positions(db->dbNodes), 2);
+  scalar("qp_matrix_hpwl", db->calcNetBoundPins());
+  QPPlacer qp(db);
+  int n = (int)db->dbNodes.size();
+  SMatrix XA(n, n), YA(n, n);
+  VectorXf Xx(n), Yx(n), Xb(n), Yb(n);
+  qp.createSparseMatrix(XA, YA, Xx, Yx, Xb, Yb);
+  dump_coo("qp_ax", XA);
+  dump_coo("qp_ay", YA);
+  npy_f32("qp_bx", vecf(Xb));
+  npy_f32("qp_by", vecf(Yb));
+  npy_f32("qp_guess_x", vecf(Xx));
+  npy_f32("qp_guess_y", vecf(Yx));
+  double t0 = now();
+  qp.quadraticPlacement();
+  scalar("qp_seconds", now() - t0);
+  npy_f32("qp_positions", positions(db->dbNodes), 2);
+  scalar("qp_hpwl", db->calcHPWL());
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -413,6 +483,7 @@ int main(int argc, char **argv) {
   if (mode == "fft") return mode_fft(argc, argv);
   if (mode == "time") return mode_time(argc, argv);
   if (mode == "legal") return mode_legal(argc, argv);
+  if (mode == "qp") return mode_qp(argc, argv);
   fprintf(stderr, "unknown mode %s\n", mode.c_str());
   return 2;
 }

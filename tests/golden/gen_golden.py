@@ -1,7 +1,7 @@
 """Generate golden fixtures from the reference placer (run in the container
 that has the reference tree; the fixtures are committed, the reference is not).
 
-    python tests/golden/gen_golden.py [gp] [poisson] [legal]   (default: all)
+    python tests/golden/gen_golden.py [gp] [poisson] [legal] [qp]   (default: all)
 
 Builds oracle/_ref/ref_harness from the reference's own sources
 (oracle/Makefile.ref), runs it on the reference's smoke design and on seeded
@@ -30,7 +30,8 @@ KEEP = ["nodes_whxy", "nodes_macro", "terms_whxyll", "terms_ni", "pin_obj", "pin
         "init_map_terminal", "init_map_base", "init_map_ex", "init_map_ey", "init_map_phi", "init_grad_density",
         "init_grad_wirelength", "init_grad_total", "it0_main", "it1_main", "it2_main", "it0_db_positions",
         "it1_db_positions", "it2_db_positions", "trajectory", "final_positions", "legal_positions",
-        "input_positions"]
+        "input_positions", "qp_ax_row", "qp_ax_col", "qp_ax_val", "qp_ay_row", "qp_ay_col", "qp_ay_val", "qp_bx",
+        "qp_by", "qp_guess_x", "qp_guess_y", "qp_matrix_positions", "qp_positions"]
 
 
 def run(args, out):
@@ -68,6 +69,8 @@ def main(parts):
         poisson_fixtures(tmp)
     if "legal" in parts:
         legal_fixtures(tmp)
+    if "qp" in parts:
+        qp_fixtures(tmp)
     shutil.rmtree(tmp)
 
 
@@ -135,5 +138,38 @@ def legal_fixtures(tmp):
         pack(name, tmp + "/out_" + name)
 
 
+def qp_log(path):
+    """The reference's per-iteration QP line (qplace.cpp:89): iteration,
+    max(X error, Y error), HPWL."""
+    rows = []
+    for line in open(path):
+        if "at iteration number" in line:
+            f = line.replace(",", " ").split()
+            rows.append((float(f[4]), float(f[7]), float(f[9])))
+    return np.array(rows, np.float64)
+
+
+def qp_fixtures(tmp):
+    # 5. QPPlacer: createSparseMatrix at the .pl positions, then
+    # quadraticPlacement from the core centre (the reference's default flow)
+    smoke = os.path.join(REF, "examples", "smoke", "smoke.aux")
+    cases = {
+        "qp_smoke": (smoke, None),
+        "qp_synth2k": (None, synthetic(2000, 2100, seed=31, num_io=100, init="random", name="qp_synth2k")),
+        "qp_mixed3k": (None, synthetic(3000, 3200, seed=32, num_io=200, num_macros=4, util=0.7, init="random",
+                                       name="qp_mixed3k")),
+    }
+    for name, (aux, d) in cases.items():
+        args = ["qp", "-noQP"]
+        if d is not None:
+            aux = write_bookshelf(d, os.path.join(tmp, name))
+            args += ["-loadpl", aux[:-4] + ".pl"]
+        run(args + ["-aux", aux, "-IPiteCount", "0"], tmp + "/out0_" + name)
+        it0 = np.load(os.path.join(tmp, "out0_" + name, "qp_positions.npy"))
+        run(args + ["-aux", aux], tmp + "/out_" + name)
+        log = qp_log(os.path.join(tempfile.gettempdir(), "ref_harness.log"))
+        pack(name, tmp + "/out_" + name, {"qp_log": log, "qp_positions_it0": it0})
+
+
 if __name__ == "__main__":
-    main(set(sys.argv[1:]) or {"gp", "poisson", "legal"})
+    main(set(sys.argv[1:]) or {"gp", "poisson", "legal", "qp"})

@@ -91,19 +91,24 @@ def test_cli_usage_and_bad_input(tmp_path):
 def test_cli_without_device_fails_cleanly(tmp_path):
     d = synthetic(num_cells=100, num_nets=120, seed=1, name="nodev")
     aux = write_bookshelf(d, str(tmp_path / "in"))
+    r = subprocess.run([exe(), "-aux", aux, "-noQP", "-outputPath", str(tmp_path / "out")], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 1
+    assert "ep_create" in r.stderr
+    # QP runs first by default (ePlace_main.cpp:121-126)
     r = subprocess.run([exe(), "-aux", aux, "-outputPath", str(tmp_path / "out")], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 1
-    assert "ep_create" in r.stderr
+    assert "ep_qplace" in r.stderr
 
 
 @pytest.mark.gpu
 def test_cli_matches_python_api(tmp_path):
-    from eplace_amd import EPlacer2D, EplaceNesterovOpt
+    from eplace_amd import EPlacer2D, EplaceNesterovOpt, quadratic_placement
 
     d = synthetic(num_cells=3000, num_nets=3200, seed=2, name="cli3k")
     aux = write_bookshelf(d, str(tmp_path / "in"))
-    print("ep_place on cli3k (3000 cells, bins 64)", flush=True)
+    print("ep_place on cli3k (3000 cells, QP, bins 64)", flush=True)
     r = subprocess.run([exe(), "-aux", aux, "-bins", "64", "-internalLegal", "-outputPath", str(tmp_path / "out")],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -111,6 +116,11 @@ def test_cli_matches_python_api(tmp_path):
     hpwl_cli = float([ln for ln in r.stdout.splitlines() if ln.startswith("Final HPWL")][0].split()[-1])
 
     e = read_bookshelf(aux)
+    qp = quadratic_placement(e)
+    assert "INFO:  initial HPWL" in r.stdout
+    qlines = [ln for ln in r.stdout.splitlines() if "at iteration number" in ln]
+    assert len(qlines) == qp["iterations"]
+    assert abs(float(qlines[-1].split()[-1]) / qp["final_hpwl"] - 1) < 1e-6
     gp = EPlacer2D(e, 1.0, bins=64)
     gp.initialization()
     res = EplaceNesterovOpt(gp).opt()
