@@ -124,6 +124,18 @@ class EPQpSystem(C.Structure):
                 ("col", C.c_void_p), ("val", C.c_void_p), ("diag", C.c_void_p), ("b", C.c_void_p)]
 
 
+class EPMlgParams(C.Structure):
+    _fields_ = [("target_density", C.c_float), ("max_outer", C.c_int32), ("max_inner", C.c_int32),
+                ("rand_seed", C.c_uint32), ("rand_skip", C.c_int64)]
+
+
+class EPMlgStats(C.Structure):
+    _fields_ = [("macros", C.c_int32), ("bins_x", C.c_int32), ("bins_y", C.c_int32), ("outer_iterations", C.c_int32),
+                ("perturbations", C.c_int64), ("accepted", C.c_int64), ("overlap_free", C.c_int32),
+                ("final_overlap", C.c_int32), ("initial_hpwl", C.c_double), ("final_hpwl", C.c_double),
+                ("cell_area_covered", C.c_double), ("rand_draws", C.c_int64), ("seconds", C.c_double)]
+
+
 SYMBOLS = {
     "ep_create": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "ep_destroy": ([C.c_void_p], C.c_int),
@@ -152,6 +164,9 @@ SYMBOLS = {
     "ep_qplace": ([C.c_void_p] * 5, C.c_int),
     "ep_qp_matrix": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p], C.c_int),
     "ep_qp_solve": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_float, C.c_int32, C.c_void_p, C.c_void_p], C.c_int),
+    "ep_mlg_last_error": ([], C.c_char_p),
+    "ep_legalize_macros": ([C.c_void_p] * 5, C.c_int),
+    "ep_macro_legalize": ([C.c_void_p] * 4, C.c_int),
     "ep_ctrl_create": ([C.c_int32, C.c_void_p, C.c_uint32, C.c_void_p], C.c_int),
     "ep_ctrl_destroy": ([C.c_void_p], C.c_int),
     "ep_ctrl_last_error": ([C.c_void_p], C.c_char_p),

@@ -16,33 +16,6 @@
 
 namespace ep {
 
-// glibc rand() (TYPE_3 additive feedback, srand(seed)): the reference places
-// fillers with unseeded rand() (placedb.cpp:353-379).
-struct Rand {
-  std::vector<uint32_t> r;
-  explicit Rand(uint32_t seed) {
-    if (seed == 0) seed = 1;
-    r.resize(344);
-    int32_t w = (int32_t)seed;
-    r[0] = (uint32_t)w;
-    for (int i = 1; i < 31; ++i) {
-      int32_t hi = w / 127773, lo = w % 127773;
-      w = 16807 * lo - 2836 * hi;
-      if (w < 0) w += 2147483647;
-      r[i] = (uint32_t)w;
-    }
-    for (int i = 31; i < 34; ++i) r[i] = r[i - 31];
-    for (int i = 34; i < 344; ++i) r[i] = r[i - 31] + r[i - 3];
-  }
-  int32_t next() {
-    size_t i = r.size();
-    uint32_t v = r[i - 31] + r[i - 3];
-    r.push_back(v);
-    if (r.size() > (1u << 20)) r.erase(r.begin(), r.end() - 64);
-    return (int32_t)(v >> 1);
-  }
-};
-
 namespace {
 
 inline bool f_less(float a, float b) { return (double)(a - b) < -kEps; }
@@ -864,6 +837,27 @@ int ep_switch_to_filler_only(ep_ctx *x) {
   c.stage = EP_STAGE_FILLERONLY;
   ep::rebuild_stage(c);
   return ep::update_scalars(c, [&](ep::DevScalars &h) { h.stage = c.stage; });
+}
+
+int ep_macro_legalize(ep_ctx *x, const ep_design *d, const ep_mlg_params *p, ep_mlg_stats *st) {
+  if (!x || !d || !p) return EP_ERR_ARG;
+  Ctx &c = x->c;
+  if (d->num_nodes != c.N) return c.err = "design does not match the placer", EP_ERR_ARG;
+  if (!(p->target_density > 0)) return c.err = "target_density must be > 0", EP_ERR_ARG;
+  int rc = ep::sync(c);
+  if (rc) return rc;
+  std::vector<float2> pos(c.N);
+  EP_HIP(hipMemcpy(pos.data(), c.pos, (size_t)c.N * sizeof(float2), hipMemcpyDeviceToHost));
+  std::vector<float> xy(2 * (size_t)c.N);
+  for (int i = 0; i < c.N; ++i) xy[2 * i] = pos[i].x, xy[2 * i + 1] = pos[i].y;
+  ep_mlg_stats s{};
+  rc = ep::macro_legalize_sa(*d, xy, *p, *c.rng, s);
+  if (rc) return c.err = "macro legalisation failed", rc;
+  for (int i = 0; i < c.N; ++i) pos[i] = make_float2(xy[2 * i], xy[2 * i + 1]);
+  EP_HIP(hipMemcpy(c.pos, pos.data(), (size_t)c.N * sizeof(float2), hipMemcpyHostToDevice));
+  c.eval_valid = false;
+  if (st) *st = s;
+  return EP_OK;
 }
 
 int ep_switch_to_cgp(ep_ctx *x, int32_t mgp_iterations) {

@@ -11,6 +11,36 @@
 
 namespace ep {
 
+// glibc rand() (TYPE_3 additive feedback after srand(seed)).  The
+// reference never seeds it: fillers (placedb.cpp:353-379) and the macro
+// legaliser's annealing (legalizer.cpp:1414-1517) draw from one stream.
+struct Rand {
+  std::vector<uint32_t> r;
+  int64_t draws = 0;
+  explicit Rand(uint32_t seed) {
+    if (seed == 0) seed = 1;
+    r.resize(344);
+    int32_t w = (int32_t)seed;
+    r[0] = (uint32_t)w;
+    for (int i = 1; i < 31; ++i) {
+      int32_t hi = w / 127773, lo = w % 127773;
+      w = 16807 * lo - 2836 * hi;
+      if (w < 0) w += 2147483647;
+      r[i] = (uint32_t)w;
+    }
+    for (int i = 31; i < 34; ++i) r[i] = r[i - 31];
+    for (int i = 34; i < 344; ++i) r[i] = r[i - 31] + r[i - 3];
+  }
+  int32_t next() {
+    size_t i = r.size();
+    uint32_t v = r[i - 31] + r[i - 3];
+    r.push_back(v);
+    if (r.size() > (1u << 20)) r.erase(r.begin(), r.end() - 64);
+    ++draws;
+    return (int32_t)(v >> 1);
+  }
+};
+
 enum ObjKind : uint8_t { KIND_CELL = 0, KIND_MACRO = 1, KIND_FILLER = 2 };
 
 constexpr double kEps = 1.0e-15;  // global.h:39
@@ -236,6 +266,11 @@ int ctrl_device(const CtrlDev *ctrl);
       return EP_ERR_HIP;                                                       \
     }                                                                          \
   } while (0)
+
+// Annealing macro legaliser (macro_legalize.cpp); xy: [N][2] node centres,
+// updated in place; draws from rng.
+int macro_legalize_sa(const ep_design &d, std::vector<float> &xy, const ep_mlg_params &p, Rand &rng,
+                      ep_mlg_stats &st);
 
 }  // namespace ep
 

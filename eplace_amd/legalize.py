@@ -1,5 +1,6 @@
-"""Standard-cell legalisation (AbacusLegalizer, Legalization/legalizer.cpp:58-680)
-and a legality checker, through the library's host entry points."""
+"""Legalisation through the library's host entry points: standard cells
+(AbacusLegalizer, Legalization/legalizer.cpp:58-680), macros
+(SAMacroLegalizer, legalizer.cpp:701-1530) and a legality checker."""
 from __future__ import annotations
 
 import ctypes as C
@@ -48,3 +49,22 @@ def check_legal(d: Design, xy=None, site_step: float | None = None) -> dict:
     out["legal"] = all(out[k] == 0 for k in ("overlaps", "off_row", "off_site", "out_of_core", "in_blockage",
                                                "macro_overlaps"))
     return out
+
+
+def legalize_macros(d: Design, xy=None, target_density: float = 1.0, rand_seed: int = 1, rand_skip: int = 0,
+                    max_outer: int = 0, max_inner: int = 0):
+    """SAMacroLegalizer(db).legalization() on node centres xy (default: the
+    design's) with a rand() stream of seed rand_seed after rand_skip draws.
+    Returns the centres after legalisation [N, 2] and the stats."""
+    lib = L.load()
+    sd, keep = design_struct(d)
+    inp = _centres(d, xy)
+    out = np.zeros_like(inp)
+    p = L.EPMlgParams(target_density=float(target_density), max_outer=int(max_outer), max_inner=int(max_inner),
+                      rand_seed=int(rand_seed), rand_skip=int(rand_skip))
+    st = L.EPMlgStats()
+    rc = lib.ep_legalize_macros(C.byref(sd), inp.ctypes.data, C.byref(p), out.ctypes.data, C.byref(st))
+    if rc != L.EP_OK:
+        raise L.EPError(rc, lib.ep_mlg_last_error().decode())
+    del keep
+    return out, {k: getattr(st, k) for k, _ in L.EPMlgStats._fields_}

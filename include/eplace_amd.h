@@ -278,6 +278,43 @@ typedef struct ep_qp_system {
   const float *b;        /* [ndim*n] */
 } ep_qp_system;
 
+/* ---------------------------------------------------------------------------
+ * Macro legalisation: SAMacroLegalizer (Legalization/legalizer.cpp:701-1530),
+ * the simulated annealing over the movable macros, on the host.  Errors are
+ * described by ep_mlg_last_error() (standalone) or ep_last_error() (placer).
+ * ------------------------------------------------------------------------- */
+typedef struct ep_mlg_params {
+  float target_density; /* SAMacroLegalizer::setTargetDensity (ePlace_main.cpp:179) */
+  int32_t max_outer;    /* jLimit; <= 0: 1000 (legalizer.cpp:1223) */
+  int32_t max_inner;    /* kLimit; <= 0: 1000 (legalizer.cpp:1224) */
+  uint32_t rand_seed;   /* srand seed of the rand() stream (standalone entry); 1 = unseeded, as the reference */
+  int64_t rand_skip;    /* rand() draws made before the annealing (standalone entry) */
+} ep_mlg_params;
+
+typedef struct ep_mlg_stats {
+  int32_t macros, bins_x, bins_y, outer_iterations;
+  int64_t perturbations, accepted;
+  int32_t overlap_free;     /* overlapFree */
+  int32_t final_overlap;    /* totalMacroOverlap: total macro area - union area */
+  double initial_hpwl, final_hpwl; /* PlaceDB::calcHPWL before / after */
+  double cell_area_covered; /* totalCellAreaCovered at the end */
+  int64_t rand_draws;       /* rand() draws the annealing made */
+  double seconds;
+} ep_mlg_stats;
+
+const char *ep_mlg_last_error(void);
+/* SAMacroLegalizer(db).legalization() on the given node centres (NULL: the
+ * design's) with its own rand() stream (rand_seed, rand_skip); out_xy
+ * receives the centres after legalisation (macros moved, cells unchanged). */
+int ep_legalize_macros(const ep_design *design, const float *node_xy, const ep_mlg_params *params, float *out_xy,
+                       ep_mlg_stats *stats);
+/* The same on a placer (ePlace_main.cpp:174-186): node positions are read
+ * from and written back to the placer, and the rand() stream is the
+ * placer's (the one its fillers were drawn from, continued by the next
+ * ep_switch_to_filler_only).  design must be the one the placer was created
+ * from. */
+int ep_macro_legalize(ep_ctx *ctx, const ep_design *design, const ep_mlg_params *params, ep_mlg_stats *stats);
+
 const char *ep_qp_last_error(void);
 /* QPPlacer::quadraticPlacement (qplace.cpp:20-103).  node_xy: [num_nodes][2]
  * centres, read when start_at_center is 0 and replaced by the result (the

@@ -22,6 +22,9 @@
 //        guess; then QPPlacer::quadraticPlacement, whose per-iteration
 //        HPWL and solver error go to stdout, and the node centres after it;
 //        -timeOnly: only the wall time and HPWL of quadraticPlacement)
+//   mlg  -aux F.aux -noQP [-centers C.f32] [-targetDensity TD] [-randSkip K] -out DIR
+//        (SAMacroLegalizer on the node centres after K rand() draws; the
+//        centres after it, its final cost terms and the next rand() value)
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -465,6 +468,44 @@ int mode_qp(int argc, char **argv) {
   return 0;
 }
 
+int mode_mlg(int argc, char **argv) {
+  BookshelfParser parser;
+  PlaceDB *db = new PlaceDB();
+  load_design(argc, argv, parser, db);
+  dump_design(db);
+  std::string cin = arg_str(argc, argv, "centers", "");
+  if (!cin.empty()) {
+    std::vector<float> xy(db->dbNodes.size() * 2);
+    FILE *f = fopen(cin.c_str(), "rb");
+    if (!f || fread(xy.data(), 4, xy.size(), f) != xy.size()) {
+      fprintf(stderr, "bad centers file\n");
+      return 2;
+    }
+    fclose(f);
+    for (size_t i = 0; i < db->dbNodes.size(); ++i) db->setModuleCenter_2D(db->dbNodes[i], xy[2 * i], xy[2 * i + 1]);
+  }
+  long long skip = atoll(arg_str(argc, argv, "randSkip", "0").c_str());
+  for (long long k = 0; k < skip; ++k) rand();
+  float td = (float)atof(arg_str(argc, argv, "targetDensity", "1.0").c_str());
+  npy_f32("input_positions", positions(db->dbNodes), 2);
+  scalar("input_hpwl", db->calcHPWL());
+  SAMacroLegalizer legal(db);
+  legal.setTargetDensity(td);
+  double t0 = now();
+  legal.legalization();
+  scalar("mlg_seconds", now() - t0);
+  npy_f32("mlg_positions", positions(db->dbNodes), 2);
+  scalar("mlg_hpwl", db->calcHPWL());
+  scalar("mlg_overlap", legal.totalMacroOverlap);
+  scalar("mlg_overlap_free", legal.overlapFree ? 1 : 0);
+  scalar("mlg_total_hpwl", legal.totalHPWL);
+  scalar("mlg_cell_covered", legal.totalCellAreaCovered);
+  scalar("mlg_bins", legal.binDimension.x);
+  scalar("mlg_macros", (double)legal.dbMacros.size());
+  scalar("mlg_rand_next", rand());
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -484,6 +525,7 @@ int main(int argc, char **argv) {
   if (mode == "time") return mode_time(argc, argv);
   if (mode == "legal") return mode_legal(argc, argv);
   if (mode == "qp") return mode_qp(argc, argv);
+  if (mode == "mlg") return mode_mlg(argc, argv);
   fprintf(stderr, "unknown mode %s\n", mode.c_str());
   return 2;
 }

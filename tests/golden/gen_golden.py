@@ -1,7 +1,7 @@
 """Generate golden fixtures from the reference placer (run in the container
 that has the reference tree; the fixtures are committed, the reference is not).
 
-    python tests/golden/gen_golden.py [gp] [poisson] [legal] [qp]   (default: all)
+    python tests/golden/gen_golden.py [gp] [poisson] [legal] [qp] [mlg]   (default: all)
 
 Builds oracle/_ref/ref_harness from the reference's own sources
 (oracle/Makefile.ref), runs it on the reference's smoke design and on seeded
@@ -31,7 +31,7 @@ KEEP = ["nodes_whxy", "nodes_macro", "terms_whxyll", "terms_ni", "pin_obj", "pin
         "init_grad_wirelength", "init_grad_total", "it0_main", "it1_main", "it2_main", "it0_db_positions",
         "it1_db_positions", "it2_db_positions", "trajectory", "final_positions", "legal_positions",
         "input_positions", "qp_ax_row", "qp_ax_col", "qp_ax_val", "qp_ay_row", "qp_ay_col", "qp_ay_val", "qp_bx",
-        "qp_by", "qp_guess_x", "qp_guess_y", "qp_matrix_positions", "qp_positions"]
+        "qp_by", "qp_guess_x", "qp_guess_y", "qp_matrix_positions", "qp_positions", "mlg_positions"]
 
 
 def run(args, out):
@@ -71,6 +71,8 @@ def main(parts):
         legal_fixtures(tmp)
     if "qp" in parts:
         qp_fixtures(tmp)
+    if "mlg" in parts:
+        mlg_fixtures(tmp)
     shutil.rmtree(tmp)
 
 
@@ -171,5 +173,30 @@ def qp_fixtures(tmp):
         pack(name, tmp + "/out_" + name, {"qp_log": log, "qp_positions_it0": it0})
 
 
+def mlg_fixtures(tmp):
+    # 6. SAMacroLegalizer on given node centres after K rand() draws: the
+    # reference's own mGP result of synth_mixed, and macro-heavy designs
+    # started at the core centre (one, two and seven annealing rounds)
+    def macro_design(seed, cells, macros, util, side=(0.03, 0.12), cap=0.45):
+        d = synthetic(cells, int(cells * 1.05), seed=seed, num_io=cells // 50, num_macros=macros, util=util,
+                      macro_side=side, macro_area_cap=cap, init="center", name=f"mlg{seed}")
+        return d, np.stack([d.node_x, d.node_y], 1)
+    mixed = synthetic(num_cells=2000, num_nets=2200, seed=13, num_macros=4, util=0.75, name="mlg_mixed")
+    cases = {
+        "mlg_mixed": (mixed, np.load(os.path.join(HERE, "synth_mixed.npz"))["final_positions"][:mixed.num_nodes],
+                      0, 1.0),
+        "mlg_m52": macro_design(52, 4000, 20, 0.7) + (777, 0.9),
+        "mlg_mm62": macro_design(62, 3000, 25, 0.6, (0.08, 0.2), 0.6) + (99, 1.0),
+        "mlg_mm63": macro_design(63, 3000, 60, 0.55, (0.04, 0.12), 0.6) + (99, 1.0),
+    }
+    for name, (d, xy, skip, td) in cases.items():
+        aux = write_bookshelf(d, os.path.join(tmp, name))
+        f = os.path.join(tmp, name + ".f32")
+        np.ascontiguousarray(xy, np.float32).tofile(f)
+        run(["mlg", "-aux", aux, "-noQP", "-centers", f, "-randSkip", str(skip), "-targetDensity", str(td)],
+            tmp + "/out_" + name)
+        pack(name, tmp + "/out_" + name, {"rand_skip": skip, "target_density": td})
+
+
 if __name__ == "__main__":
-    main(set(sys.argv[1:]) or {"gp", "poisson", "legal", "qp"})
+    main(set(sys.argv[1:]) or {"gp", "poisson", "legal", "qp", "mlg"})
