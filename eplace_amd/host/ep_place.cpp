@@ -10,9 +10,9 @@
 // core centre, ePlace_main.cpp:121-126; -noQP keeps the .pl or -loadpl
 // positions), mGP with EplaceNesterovOpt and writes
 // <outputPath>/<name>/<name>-eGP.pl (lower-left corners, as
-// PlaceDB::outputBookShelf does, placedb.cpp:1022-1060).  For designs with
-// macros, mLG + FILLERONLY + cGP are reported and skipped (as with -nomLG).
-// Logging goes to stdout.
+// PlaceDB::outputBookShelf does, placedb.cpp:1022-1060).  Designs with macros
+// then run mLG (SAMacroLegalizer), FILLERONLY and cGP (ePlace_main.cpp:174-219)
+// unless -nomLG / -nocGP.  Logging goes to stdout.
 // -internalLegal runs the Abacus legaliser on the result (reference
 // semantics, ePlace_main.cpp:248-259), checks it and writes
 // <name>-legal.pl beside the global placement.
@@ -36,14 +36,15 @@ namespace {
 struct Args {
   std::string aux, loadpl, out = ".", dump;
   float td = 1.0f, tof = 0.10f;
-  bool bktrk = false, lse = false, no_mgp = false, legal = false, no_qp = false;
+  bool bktrk = false, lse = false, no_mgp = false, legal = false, no_qp = false, no_mlg = false, no_cgp = false;
   int bins = 0, qp_iters = 20;
 };
 
 int usage(const char *prog) {
   fprintf(stderr,
           "usage: %s -aux <design.aux> [-targetDensity d] [-targetOverflow t] [-loadpl f.pl]\n"
-          "          [-noQP] [-IPiteCount n] [-bktrk] [-LSE] [-bins n] [-nomGP] [-internalLegal]\n"
+          "          [-noQP] [-IPiteCount n] [-bktrk] [-LSE] [-bins n] [-nomGP] [-nomLG] [-nocGP]\n"
+          "          [-internalLegal]\n"
           "          [-outputPath dir] [-dumpDesign file]\n",
           prog);
   return 2;
@@ -88,7 +89,11 @@ bool parse(int argc, char **argv, Args &a) {
       a.qp_iters = atoi(v.c_str());
     } else if (k == "-noQP") {
       a.no_qp = true;
-    } else if (k == "-nomLG" || k == "-nocGP" || k == "-noLegal") {
+    } else if (k == "-nomLG") {
+      a.no_mlg = true;
+    } else if (k == "-nocGP") {
+      a.no_cgp = true;
+    } else if (k == "-noLegal") {
       // these stages are not run here; accepted for command-line compatibility
     } else {
       fprintf(stderr, "unknown option %s\n", k.c_str());
@@ -176,7 +181,28 @@ int main(int argc, char **argv) {
     size_t macros = 0;
     for (float h : d.node_h)  // PlaceDB::addNode (placedb.cpp:102-111)
       macros += (double)(h - (float)view.common_row_height) > 1e-15;
-    if (macros > 0) printf("%zu macros: mLG, FILLERONLY and cGP are not run\n", macros);
+    if (macros > 0 && !a.no_mlg) {
+      // ePlace_main.cpp:174-219
+      SAMacroLegalizer mlg(&placer, view);
+      mlg.setTargetDensity(a.td);
+      ep_mlg_stats ms = mlg.legalization();
+      printf("mLG finished: %d macros, %d rounds, %lld moves, overlap %d; HPWL after mLG: %.6e (%.3f s)\n", ms.macros,
+             ms.outer_iterations, (long long)ms.perturbations, ms.final_overlap, ms.final_hpwl, ms.seconds);
+      if (!a.no_cgp) {
+        placer.switch2FillerOnly();
+        EplaceNesterovOpt<VECTOR_3D> fo(&placer, a.bktrk, a.tof);
+        fo.opt();
+        printf("filler placement finished: %d iterations\n", fo.last().iterations);
+        placer.switch2cGP();
+        EplaceNesterovOpt<VECTOR_3D> cg(&placer, a.bktrk, a.tof);
+        cg.opt();
+        const ep_run_stats &s = cg.last();
+        printf("cGP finished: %d iterations, overflow %.4f\n", s.iterations, s.final_overflow);
+        printf("cGP Final HPWL: %.6e\n", s.final_hpwl);
+      }
+    } else if (macros > 0) {
+      printf("%zu macros: mLG skipped (-nomLG)\n", macros);
+    }
 
     std::vector<float> xy(2 * d.node_w.size());
     std::vector<float> all(2 * (size_t)info.num_objects);

@@ -162,6 +162,28 @@ class QPPlacer {
   int device_;
 };
 
+// SAMacroLegalizer (Legalization/legalizer.h:304-420) on a placer, as
+// ePlace_main.cpp:174-186 runs it: positions and the rand() stream are the
+// placer's.
+class SAMacroLegalizer {
+ public:
+  SAMacroLegalizer(EPlacer_2D *placer, const ep_design &design) : placer_(placer), design_(design) {}
+  void setTargetDensity(float td) { targetDensity_ = td; }
+  ep_mlg_stats legalization() {
+    ep_mlg_params p{};
+    p.target_density = targetDensity_;
+    p.rand_seed = 1;
+    ep_mlg_stats st{};
+    placer_->check(ep_macro_legalize(placer_->context(), &design_, &p, &st));
+    return st;
+  }
+
+ private:
+  EPlacer_2D *placer_;
+  ep_design design_;
+  float targetDensity_ = 1.0f;
+};
+
 template <typename T>
 class FirstOrderOptimizer {
  public:

@@ -68,3 +68,29 @@ def legalize_macros(d: Design, xy=None, target_density: float = 1.0, rand_seed: 
         raise L.EPError(rc, lib.ep_mlg_last_error().decode())
     del keep
     return out, {k: getattr(st, k) for k, _ in L.EPMlgStats._fields_}
+
+
+class SAMacroLegalizer:
+    """SAMacroLegalizer (Legalization/legalizer.h:304-420) on a placer, as the
+    reference runs it between mGP and FILLERONLY (ePlace_main.cpp:174-186):
+    node positions are read from and written back to the placer and the
+    annealing draws from the placer's rand() stream."""
+
+    def __init__(self, placer):
+        self.placer = placer
+        self.targetDensity = 1.0
+        self.stats = None
+
+    def setTargetDensity(self, td: float) -> None:
+        self.targetDensity = float(td)
+
+    def legalization(self, max_outer: int = 0, max_inner: int = 0) -> dict:
+        pl = self.placer
+        sd, keep = design_struct(pl.design)
+        p = L.EPMlgParams(target_density=self.targetDensity, max_outer=int(max_outer), max_inner=int(max_inner),
+                          rand_seed=1, rand_skip=0)
+        st = L.EPMlgStats()
+        pl._check(pl.lib.ep_macro_legalize(pl.h, C.byref(sd), C.byref(p), C.byref(st)))
+        del keep
+        self.stats = {k: getattr(st, k) for k, _ in L.EPMlgStats._fields_}
+        return self.stats

@@ -145,6 +145,13 @@ class EPlacer2D:
     def updatePenaltyFactor(self):
         self._check(self.lib.ep_update_penalty_factor(self.h))
 
+    def calcHPWL(self) -> float:
+        """HPWL of the current positions (PlaceDB::calcHPWL, placedb.cpp:424-435)."""
+        import ctypes
+        v = ctypes.c_double()
+        self._check(self.lib.ep_hpwl(self.h, ctypes.byref(v)))
+        return v.value
+
     def switch2FillerOnly(self):
         self._check(self.lib.ep_switch_to_filler_only(self.h))
         self.placementStage = FILLERONLY

@@ -9,7 +9,8 @@
 //
 // Modes
 //   gp   -aux F.aux -targetDensity TD [-targetOverflow T] [-bktrk] [-LSE]
-//        [-noQP] [-loadpl F.pl] [-iters K] [-dumpEvery M] [-legal] -out DIR
+//        [-noQP] [-loadpl F.pl] [-iters K] [-dumpEvery M] [-mixed] [-legal] -out DIR
+//        (-mixed: for designs with macros, mLG, FILLERONLY and cGP after mGP)
 //   fft  -nx NX -ny NY -bx BX -by BY -in RHO.f32 -out DIR
 //   time -aux F.aux -targetDensity TD [-noQP] [-loadpl F.pl] [-iters K]
 //        (wall time of K vanilla iterations after initialisation, 1 thread)
@@ -312,6 +313,31 @@ int mode_gp(int argc, char **argv) {
   opt.wrap_up();
   npy_f32("final_positions", positions(ep->ePlaceNodesAndFillers), 2);
   scalar("final_hpwl", db->calcHPWL());
+  if (arg_flag(argc, argv, "mixed") && db->dbMacroCount > 0) {
+    // ePlace_main.cpp:174-219: mLG, then FILLERONLY and cGP with the same
+    // optimiser object
+    SAMacroLegalizer mlg(db);
+    mlg.setTargetDensity(td);
+    mlg.legalization();
+    npy_f32("mixed_mlg_positions", positions(db->dbNodes), 2);
+    scalar("mixed_mlg_hpwl", db->calcHPWL());
+    scalar("mixed_mgp_iteration_count", ep->mGPIterationCount);
+    ep->switch2FillerOnly();
+    npy_f32("mixed_fo_fillers", positions(ep->ePlaceFillers), 2);
+    opt.opt();
+    npy_f32("mixed_fo_positions", positions(ep->ePlaceNodesAndFillers), 2);
+    scalar("mixed_fo_iterations", opt.iter_count);
+    scalar("mixed_fo_overflow", ep->globalDensityOverflow);
+    scalar("mixed_fo_lambda", ep->lambda);
+    ep->switch2cGP();
+    scalar("mixed_cgp_lambda_start", ep->lambda);
+    opt.opt();
+    npy_f32("mixed_cgp_positions", positions(ep->ePlaceNodesAndFillers), 2);
+    scalar("mixed_cgp_iterations", opt.iter_count);
+    scalar("mixed_cgp_overflow", ep->globalDensityOverflow);
+    scalar("mixed_cgp_lambda", ep->lambda);
+    scalar("mixed_cgp_hpwl", db->calcHPWL());
+  }
   if (arg_flag(argc, argv, "legal")) {
     AbacusLegalizer legal(db);
     legal.legalization();

@@ -1,7 +1,7 @@
 """Generate golden fixtures from the reference placer (run in the container
 that has the reference tree; the fixtures are committed, the reference is not).
 
-    python tests/golden/gen_golden.py [gp] [poisson] [legal] [qp] [mlg]   (default: all)
+    python tests/golden/gen_golden.py [gp] [poisson] [legal] [qp] [mlg] [flow]   (default: all)
 
 Builds oracle/_ref/ref_harness from the reference's own sources
 (oracle/Makefile.ref), runs it on the reference's smoke design and on seeded
@@ -31,7 +31,8 @@ KEEP = ["nodes_whxy", "nodes_macro", "terms_whxyll", "terms_ni", "pin_obj", "pin
         "init_grad_wirelength", "init_grad_total", "it0_main", "it1_main", "it2_main", "it0_db_positions",
         "it1_db_positions", "it2_db_positions", "trajectory", "final_positions", "legal_positions",
         "input_positions", "qp_ax_row", "qp_ax_col", "qp_ax_val", "qp_ay_row", "qp_ay_col", "qp_ay_val", "qp_bx",
-        "qp_by", "qp_guess_x", "qp_guess_y", "qp_matrix_positions", "qp_positions", "mlg_positions"]
+        "qp_by", "qp_guess_x", "qp_guess_y", "qp_matrix_positions", "qp_positions", "mlg_positions", "mixed_mlg_positions", "mixed_fo_fillers",
+        "mixed_fo_positions", "mixed_cgp_positions"]
 
 
 def run(args, out):
@@ -73,6 +74,8 @@ def main(parts):
         qp_fixtures(tmp)
     if "mlg" in parts:
         mlg_fixtures(tmp)
+    if "flow" in parts:
+        flow_fixtures(tmp)
     shutil.rmtree(tmp)
 
 
@@ -198,5 +201,15 @@ def mlg_fixtures(tmp):
         pack(name, tmp + "/out_" + name, {"rand_skip": skip, "target_density": td})
 
 
+def flow_fixtures(tmp):
+    # 7. the mixed-size flow after mGP (ePlace_main.cpp:174-219): mLG,
+    # FILLERONLY and cGP, from the .pl (no QP)
+    d = synthetic(3000, 3200, seed=71, num_macros=10, util=0.7, name="flow_mixed")
+    aux = write_bookshelf(d, os.path.join(tmp, "flow_mixed"))
+    run(["gp", "-aux", aux, "-targetDensity", "1.0", "-targetOverflow", "0.1", "-noQP", "-loadpl", aux[:-4] + ".pl",
+         "-mixed"], tmp + "/out_flow_mixed")
+    pack("flow_mixed", tmp + "/out_flow_mixed", {"target_density": 1.0})
+
+
 if __name__ == "__main__":
-    main(set(sys.argv[1:]) or {"gp", "poisson", "legal", "qp", "mlg"})
+    main(set(sys.argv[1:]) or {"gp", "poisson", "legal", "qp", "mlg", "flow"})
