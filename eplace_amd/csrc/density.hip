@@ -405,12 +405,12 @@ __global__ void __launch_bounds__(kBlock) gather_big_kernel(const int32_t *__res
 // |g - g_prev|^2 and |y - y_prev|^2 over the stage's objects
 // (optutils.hpp:19-35) with, for the step controller, |grad WL|^2 and
 // |grad D|^2 over the stage.  Blocks walk the objects XCD-contiguously: with
-// nets stored by their lowest node, the pin gradients a range of nodes
-// reads lie in a narrow band of the pin array.
+// nets stored by degree class and then by their lowest node, the pin
+// gradients a range of nodes reads lie in one narrow band per class.
 __global__ void __launch_bounds__(kBlock) gather_combine_kernel(
     int O, int N, const float2 *__restrict__ pos, const uint8_t *__restrict__ kind,
     const int32_t *__restrict__ npins, const float *__restrict__ area, const int32_t *__restrict__ node_pin_start,
-    const float2 *__restrict__ pin_grad, const float2 *gd, float2 *gwl,
+    const int32_t *__restrict__ node_pin_lane, const float2 *__restrict__ pin_grad, const float2 *gd, float2 *gwl,
     float2 *gtot, int recompute, int lipschitz, const uint8_t *__restrict__ in_stage,
     const float2 *__restrict__ ref_now, const float2 *__restrict__ ref_prev,
     const float2 *__restrict__ grad_prev, double *partials, const DevScalars *sc) {
@@ -425,7 +425,7 @@ __global__ void __launch_bounds__(kBlock) gather_combine_kernel(
       if (recompute) {
         int s = node_pin_start[o], e = node_pin_start[o + 1];
         for (int q = s; q < e; ++q) {
-          float2 pg = pin_grad[q];
+          float2 pg = pin_grad[node_pin_lane[q]];
           wl.x += pg.x;
           wl.y += pg.y;
         }
@@ -578,22 +578,53 @@ void launch_finalize_density(Ctx &c) {
                      c.partials, c.sc);
 }
 
-void launch_gather_combine(Ctx &c, bool recompute, bool lipschitz, const float2 *ref_now,
-                           const float2 *ref_prev, const float2 *grad_prev) {
+// Node wirelength gradients: each node's pin gradients, read through
+// node_pin_lane in modulePins order (objects.cpp:380-415 summed per module,
+// eplace.cpp:694-737).  Runs on the wiring stream, off the critical path.
+__global__ void __launch_bounds__(kBlock) node_wl_kernel(int N, const int32_t *__restrict__ node_pin_start,
+                                                          const int32_t *__restrict__ node_pin_lane,
+                                                          const float2 *__restrict__ pin_grad, float2 *gwl,
+                                                          const DevScalars *sc) {
+  if (sc->done) return;
+  const int o = xcd_block() * blockDim.x + threadIdx.x;
+  if (o >= N) return;
+  float2 wl = make_float2(0.f, 0.f);
+  for (int q = node_pin_start[o], e = node_pin_start[o + 1]; q < e; ++q) {
+    const float2 pg = pin_grad[node_pin_lane[q]];
+    wl.x += pg.x;
+    wl.y += pg.y;
+  }
+  gwl[o] = wl;
+}
+
+void launch_node_wl(Ctx &c, hipStream_t st) {
+  hipLaunchKernelGGL(node_wl_kernel, dim3(object_blocks(c.N)), dim3(kBlock), 0, st, c.N, c.node_pin_start,
+                     c.node_pin_lane, c.pin_grad, c.gwl, c.sc);
+}
+
+void launch_field_gather(Ctx &c) {
   Grid g = c.grid();
   int ldp = poisson_ld(c);
-  if (recompute) {
-    TileSort &t = c.tiles;
-    hipLaunchKernelGGL(gather_tile_kernel, dim3(t.grid), dim3(kBlock), 0, c.stream, t.chunks, t.nchunks, t.sobj,
-                       t.svals, g, t.tyc, t.ntiles, ldp, c.ex, c.ey, c.gd, c.sc);
-    if (c.num_big)
-      hipLaunchKernelGGL(gather_big_kernel, dim3(c.num_big), dim3(kBlock), 0, c.stream, c.big, c.pos, c.w, c.h,
-                         g, ldp, c.ex, c.ey, c.gd, c.sc);
-  }
+  TileSort &t = c.tiles;
+  hipLaunchKernelGGL(gather_tile_kernel, dim3(t.grid), dim3(kBlock), 0, c.stream, t.chunks, t.nchunks, t.sobj,
+                     t.svals, g, t.tyc, t.ntiles, ldp, c.ex, c.ey, c.gd, c.sc);
+  if (c.num_big)
+    hipLaunchKernelGGL(gather_big_kernel, dim3(c.num_big), dim3(kBlock), 0, c.stream, c.big, c.pos, c.w, c.h, g,
+                       ldp, c.ex, c.ey, c.gd, c.sc);
+}
+
+void launch_combine(Ctx &c, bool recompute, bool lipschitz, const float2 *ref_now, const float2 *ref_prev,
+                    const float2 *grad_prev) {
   hipLaunchKernelGGL(gather_combine_kernel, dim3(object_blocks(c.O)), dim3(kBlock), 0, c.stream, c.O, c.N,
-                     c.pos, c.kind, c.npins, c.area, c.node_pin_start, c.pin_grad, c.gd, c.gwl,
+                     c.pos, c.kind, c.npins, c.area, c.node_pin_start, c.node_pin_lane, c.pin_grad, c.gd, c.gwl,
                      c.gtot, recompute ? 1 : 0, lipschitz ? 1 : 0, c.in_stage, ref_now, ref_prev, grad_prev,
                      c.partials, c.sc);
+}
+
+void launch_gather_combine(Ctx &c, bool recompute, bool lipschitz, const float2 *ref_now,
+                           const float2 *ref_prev, const float2 *grad_prev) {
+  if (recompute) launch_field_gather(c);
+  launch_combine(c, recompute, lipschitz, ref_now, ref_prev, grad_prev);
 }
 
 void launch_lambda_init(Ctx &c) {

@@ -154,6 +154,9 @@ int create(Ctx &c, const ep_design *d, const ep_params *p) {
   c.device = p->device;
   EP_HIP(hipSetDevice(c.device));
   EP_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  EP_HIP(hipStreamCreateWithFlags(&c.stream2, hipStreamNonBlocking));
+  EP_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+  EP_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
   for (auto &e : c.ev) EP_HIP(hipEventCreate(&e));
   c.td = p->target_density;
   c.wl_model = p->wirelength_model;
@@ -348,7 +351,23 @@ int create(Ctx &c, const ep_design *d, const ep_params *p) {
       cnt[k + 1]++;
     }
     for (int k = 0; k <= c.N; ++k) cnt[k + 1] += cnt[k];
-    for (int e = 0; e < c.E; ++e) order[cnt[key[e]]++] = e;
+    std::vector<int32_t> by_node(c.E);
+    for (int e = 0; e < c.E; ++e) by_node[cnt[key[e]]++] = e;
+    // then stable by degree class: nets of 0-1 pins (no HPWL, no gradient),
+    // then 2, 3-4, 5-8, 9-16, 17-32 pins, then the larger ones
+    std::vector<int32_t> dcnt(9, 0);
+    auto dkey = [&](int e) {
+      const int64_t dg = d->net_start[e + 1] - d->net_start[e];
+      if (dg <= 1) return 0;
+      if (dg > kNetLaneMax) return 6;
+      int k = 1;
+      while ((2 << (k - 1)) < dg) ++k;  // 2 -> 1, 3-4 -> 2, 5-8 -> 3, 9-16 -> 4, 17-32 -> 5
+      return k;
+    };
+    for (int e = 0; e < c.E; ++e) dcnt[dkey(e) + 1]++;
+    for (int k = 0; k < 8; ++k) dcnt[k + 1] += dcnt[k];
+    for (int k = 0; k < 5; ++k) c.wl_first[k] = dcnt[k + 1], c.wl_count[k] = dcnt[k + 2] - dcnt[k + 1];
+    for (int e : by_node) order[dcnt[dkey(e)]++] = e;
   }
   std::vector<int32_t> ns32(c.E + 1), pobj(std::max<int64_t>(c.P, 1)), pin_net(std::max<int64_t>(c.P, 1));
   std::vector<float> pdx(std::max<int64_t>(c.P, 1)), pdy(std::max<int64_t>(c.P, 1));
@@ -369,39 +388,59 @@ int create(Ctx &c, const ep_design *d, const ep_params *p) {
     }
     ns32[c.E] = q2;
   }
-  // degree > 64 get a wave each; runs of consecutive smaller nets are packed
-  // into groups of at most 64 pins (one wave, one pin per lane)
-  std::vector<int32_t> large, gbeg, gend;
+  // degree > kNetLaneMax (stored last) get a wave each; the others fill
+  // 64-lane groups, D = 2 << c lanes per net of class c
+  std::vector<int32_t> large;
   {
-    int32_t cur_b = -1, cur_e = -1;
-    for (int e = 0; e < c.E; ++e) {
-      int32_t s0 = ns32[e], s1 = ns32[e + 1];
-      if (s1 == s0) continue;
-      if (s1 - s0 > kSmallNetMaxDegree) {
-        large.push_back(e);
-        if (cur_b >= 0) { gbeg.push_back(cur_b); gend.push_back(cur_e); cur_b = -1; }
-        continue;
-      }
-      if (cur_b >= 0 && (s1 - cur_b > 64 || s0 != cur_e)) { gbeg.push_back(cur_b); gend.push_back(cur_e); cur_b = -1; }
-      if (cur_b < 0) cur_b = s0;
-      cur_e = s1;
+    const int n_lane = c.wl_first[4] + c.wl_count[4];
+    for (int e = n_lane; e < c.E; ++e) large.push_back(e);
+    int g = 0;
+    for (int k = 0; k < 5; ++k) {
+      c.wl_gstart[k] = g;
+      g += (c.wl_count[k] * (2 << k) + 63) / 64;
     }
-    if (cur_b >= 0) { gbeg.push_back(cur_b); gend.push_back(cur_e); }
+    c.wl_gstart[5] = g;
+    c.wl_groups = g;
   }
-  c.num_groups = (int)gbeg.size();
   c.num_large = (int)large.size();
-  // ---- pin gradient slots: node by node, each node's pins in modulePins
-  // (file) order, so the node sum reads one contiguous run ----
-  std::vector<int32_t> nps(c.N + 1, 0), slot(std::max<int64_t>(c.P, 1), -1);
+  // ---- pin gradient entries: the lane groups' pins in lane order, then the
+  // pins of the large nets (stored last, contiguous); node_pin_lane lists,
+  // node by node in modulePins (file) order, the entry of each node pin ----
+  const size_t nlanes = std::max<size_t>((size_t)c.wl_groups * 64, 1);
+  std::vector<int32_t> lane_obj(nlanes, -1), lane_of_pin(std::max<int64_t>(c.P, 1), -1);
+  std::vector<float2> lane_off(nlanes, make_float2(0.f, 0.f));
+  for (int k = 0; k < 5; ++k) {
+    const int D = 2 << k;
+    for (int j = 0; j < c.wl_count[k]; ++j) {
+      const int e = c.wl_first[k] + j;
+      const size_t base = (size_t)c.wl_gstart[k] * 64 + (size_t)j * D;
+      for (int32_t p = ns32[e]; p < ns32[e + 1]; ++p) {
+        const size_t l = base + (p - ns32[e]);
+        lane_obj[l] = pobj[p];
+        lane_off[l] = make_float2(pdx[p], pdy[p]);
+        lane_of_pin[p] = (int32_t)l;
+      }
+    }
+  }
+  {
+    const int32_t p_large0 = ns32[c.wl_first[4] + c.wl_count[4]];
+    c.large_lane_off = (int32_t)((int64_t)c.wl_groups * 64 - p_large0);
+    for (int32_t p = p_large0; p < ns32[c.E]; ++p) lane_of_pin[p] = p + c.large_lane_off;
+    c.grad_lanes = (int64_t)c.wl_groups * 64 + (ns32[c.E] - p_large0);
+    // pins of one-pin nets (no gradient) read the extra entry, kept at zero
+    for (int64_t p = 0; p < c.P; ++p)
+      if (lane_of_pin[p] < 0) lane_of_pin[p] = (int32_t)c.grad_lanes;
+  }
+  std::vector<int32_t> nps(c.N + 1, 0), npl(std::max<int64_t>(c.P, 1), -1);
   for (int i = 0; i < c.N; ++i) nps[i + 1] = nps[i] + s.npins[i];
-  c.h_grad_slot.assign(c.P, -1);
+  c.h_grad_lane.assign(c.P, -1);
   {
     std::vector<int32_t> fill(nps.begin(), nps.end() - 1);
     for (int64_t q = 0; q < c.P; ++q)
       if (d->pin_obj[q] < c.N) {
-        const int32_t sl = fill[d->pin_obj[q]]++;
-        slot[pin_of[q]] = sl;
-        c.h_grad_slot[q] = sl;
+        const int32_t ln = lane_of_pin[pin_of[q]];
+        npl[fill[d->pin_obj[q]]++] = ln;
+        c.h_grad_lane[q] = ln;
       }
   }
   c.h_w = s.w;
@@ -417,12 +456,13 @@ int create(Ctx &c, const ep_design *d, const ep_params *p) {
             dev_upload(c, c.big, big.data(), big.size()) && dev_upload(c, c.is_big, is_big.data(), c.O) &&
             dev_upload(c, c.net_start, ns32.data(), c.E + 1) && dev_upload(c, c.pin_obj, pobj.data(), c.P) &&
             dev_upload(c, c.pin_dx, pdx.data(), c.P) && dev_upload(c, c.pin_dy, pdy.data(), c.P) &&
-            dev_upload(c, c.group_begin, gbeg.data(), gbeg.size()) &&
-            dev_upload(c, c.group_end, gend.data(), gend.size()) &&
+
             dev_upload(c, c.pin_net, pin_net.data(), pin_net.size()) &&
+            dev_upload(c, c.lane_obj, lane_obj.data(), nlanes) && dev_upload(c, c.lane_off, lane_off.data(), nlanes) &&
             dev_upload(c, c.large_nets, large.data(), large.size()) &&
             dev_upload(c, c.node_pin_start, nps.data(), c.N + 1) &&
-            dev_upload(c, c.pin_slot, slot.data(), slot.size()) && dev_zero(c, c.pin_grad, (size_t)nps[c.N]) &&
+            dev_upload(c, c.node_pin_lane, npl.data(), npl.size()) &&
+            dev_zero(c, c.pin_grad, (size_t)c.grad_lanes + 1) &&
             dev_upload(c, c.bllx, s.bllx.data(), c.nx) && dev_upload(c, c.burx, s.burx.data(), c.nx) &&
             dev_upload(c, c.blly, s.blly.data(), c.ny) && dev_upload(c, c.bury, s.bury.data(), c.ny) &&
             dev_zero(c, c.node_q, nb) && dev_zero(c, c.filler_q, nb) &&
@@ -454,8 +494,8 @@ void destroy(Ctx &c) {
   poisson_destroy(c);
   tile_sort_destroy(c);
   void *ptrs[] = {c.pos, c.w, c.h, c.area, c.kind, c.in_stage, c.npins, c.big, c.is_big,
-                  c.net_start, c.pin_obj, c.pin_dx, c.pin_dy, c.group_begin, c.group_end, c.pin_net, c.large_nets, c.node_pin_start,
-                  c.pin_slot, c.pin_grad, c.bllx, c.burx, c.blly, c.bury, c.node_q, c.filler_q, c.term_map,
+                  c.net_start, c.pin_obj, c.pin_dx, c.pin_dy, c.pin_net, c.lane_obj, c.lane_off, c.large_nets, c.node_pin_start,
+                  c.node_pin_lane, c.pin_grad, c.bllx, c.burx, c.blly, c.bury, c.node_q, c.filler_q, c.term_map,
                   c.base_map, c.rho, c.phi, c.ex, c.ey, c.gd, c.gwl, c.gtot, c.cur_main, c.cur_ref, c.last_ref,
                   c.last_grad, c.new_main, c.new_ref, c.cur_grad, c.sc, c.partials, c.tickets};
   for (void *p : ptrs)
@@ -463,19 +503,28 @@ void destroy(Ctx &c) {
   for (auto &e : c.ev)
     if (e) (void)hipEventDestroy(e);
   for (auto &e : c.prof_events) (void)hipEventDestroy(e);
+  if (c.ev_fork) (void)hipEventDestroy(c.ev_fork);
+  if (c.ev_join) (void)hipEventDestroy(c.ev_join);
+  if (c.stream2) {
+    (void)hipStreamSynchronize(c.stream2);
+    (void)hipStreamDestroy(c.stream2);
+  }
   if (c.stream) (void)hipStreamDestroy(c.stream);
   delete c.rng;
 }
 
-// One full evaluation of totalGradientUpdate at the current positions.
+// One full evaluation of totalGradientUpdate at the current positions.  The
+// wiring kernels and the node sums of their pin gradients need only gamma
+// (set by the density finalise), so they run on the second stream beside
+// the Poisson solve and the field gather; the combine waits for them.
 void evaluate(Ctx &c, bool lambda_update, bool lipschitz, const float2 *ref_now, const float2 *ref_prev,
               const float2 *grad_prev, bool timed) {
   hipEvent_t *ev = nullptr;
   if (timed) {
     ev = c.ev;
-  } else if (c.profiling && c.prof_used + 6 <= c.prof_events.size()) {
+  } else if (c.profiling && c.prof_used + kEvPerEval <= c.prof_events.size()) {
     ev = &c.prof_events[c.prof_used];
-    c.prof_used += 6;
+    c.prof_used += kEvPerEval;
   }
   if (ev) (void)hipEventRecord(ev[0], c.stream);
   launch_zero_maps(c);
@@ -483,14 +532,33 @@ void evaluate(Ctx &c, bool lambda_update, bool lipschitz, const float2 *ref_now,
   if (ev) (void)hipEventRecord(ev[1], c.stream);
   launch_finalize_density(c);
   if (ev) (void)hipEventRecord(ev[2], c.stream);
-  launch_wirelength(c, true);
-  launch_reduce_wirelength(c, lambda_update ? 1 : 0);
-  if (ev) (void)hipEventRecord(ev[3], c.stream);
+  (void)hipEventRecord(c.ev_fork, c.stream);
+  (void)hipStreamWaitEvent(c.stream2, c.ev_fork, 0);
+  if (ev) (void)hipEventRecord(ev[6], c.stream2);
+  launch_wirelength(c, true, c.stream2);
+  launch_reduce_wirelength(c, lambda_update ? 1 : 0, c.stream2);
+  launch_node_wl(c, c.stream2);
+  if (ev) (void)hipEventRecord(ev[7], c.stream2);
+  (void)hipEventRecord(c.ev_join, c.stream2);
   launch_poisson(c, false);
+  if (ev) (void)hipEventRecord(ev[3], c.stream);
+  launch_field_gather(c);
   if (ev) (void)hipEventRecord(ev[4], c.stream);
-  launch_gather_combine(c, true, lipschitz, ref_now, ref_prev, grad_prev);
+  (void)hipStreamWaitEvent(c.stream, c.ev_join, 0);
+  launch_combine(c, false, lipschitz, ref_now, ref_prev, grad_prev);
   launch_count_eval(c);
   if (ev) (void)hipEventRecord(ev[5], c.stream);
+}
+
+// Phase times of one evaluation from its events: scatter, finalise, wiring,
+// Poisson, field gather + combine (the last includes any wait for the wiring).
+void phase_times(const hipEvent_t *ev, double *out) {
+  static const int pairs[5][2] = {{0, 1}, {1, 2}, {6, 7}, {2, 3}, {3, 5}};
+  for (int k = 0; k < 5; ++k) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ev[pairs[k][0]], ev[pairs[k][1]]);
+    out[k] = ms * 1e-3;
+  }
 }
 
 int sync(Ctx &c) {
@@ -583,8 +651,8 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
   }
   if (o.phase_seconds) {
     // events for every evaluation, created before the timed loop
-    size_t need = (size_t)6 * ((size_t)o.max_iterations + 2) * (o.method == 1 ? 3 : 1);
-    need = std::min<size_t>(need, 6 * 20000);
+    size_t need = (size_t)kEvPerEval * ((size_t)o.max_iterations + 2) * (o.method == 1 ? 3 : 1);
+    need = std::min<size_t>(need, (size_t)kEvPerEval * 20000);
     while (c.prof_events.size() < need) {
       hipEvent_t e;
       EP_HIP(hipEventCreate(&e));
@@ -698,13 +766,12 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
   if (c.profiling) {
     c.profiling = false;
     double acc[5] = {0, 0, 0, 0, 0};
-    size_t n = c.prof_used / 6;
-    for (size_t e = 0; e < n; ++e)
-      for (int k = 0; k < 5; ++k) {
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, c.prof_events[6 * e + k], c.prof_events[6 * e + k + 1]);
-        acc[k] += ms * 1e-3;
-      }
+    size_t n = c.prof_used / kEvPerEval;
+    for (size_t e = 0; e < n; ++e) {
+      double t[5];
+      ep::phase_times(&c.prof_events[kEvPerEval * e], t);
+      for (int k = 0; k < 5; ++k) acc[k] += t[k];
+    }
     for (int k = 0; k < 5; ++k) o.phase_seconds[k] = acc[k];
     o.phase_seconds[5] = (double)n;
   }
@@ -785,11 +852,7 @@ int ep_total_gradient_update(ep_ctx *x) {
   int rc = ep::sync(c);
   if (rc) return rc;
   c.eval_valid = true;
-  for (int k = 0; k < 5; ++k) {
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, c.ev[k], c.ev[k + 1]);
-    c.last_timings[k] = ms * 1e-3;
-  }
+  ep::phase_times(c.ev, c.last_timings);
   return EP_OK;
 }
 
@@ -971,12 +1034,10 @@ int ep_read(ep_ctx *x, int32_t which, void *host, int64_t count) {
     case EP_ARR_BASE_MAP: return plain(c.base_map, nb, 4);
     case EP_ARR_PIN_GRAD: {  // back to design pin order; terminal pins read 0
       if (count != 2 * c.P) return c.err = "count mismatch", EP_ERR_ARG;
-      int64_t n = 0;
-      for (int32_t sl : c.h_grad_slot) n += sl >= 0;
-      std::vector<float2> g(std::max<int64_t>(n, 1));
-      EP_HIP(hipMemcpy(g.data(), c.pin_grad, (size_t)n * sizeof(float2), hipMemcpyDeviceToHost));
+      std::vector<float2> g((size_t)c.grad_lanes + 1);
+      EP_HIP(hipMemcpy(g.data(), c.pin_grad, g.size() * sizeof(float2), hipMemcpyDeviceToHost));
       for (int64_t q = 0; q < c.P; ++q)
-        ((float2 *)host)[q] = c.h_grad_slot[q] >= 0 ? g[c.h_grad_slot[q]] : make_float2(0.f, 0.f);
+        ((float2 *)host)[q] = c.h_grad_lane[q] >= 0 ? g[c.h_grad_lane[q]] : make_float2(0.f, 0.f);
       return EP_OK;
     }
     case EP_ARR_MAIN: return plain(c.cur_main, 2 * (int64_t)c.O, 4);

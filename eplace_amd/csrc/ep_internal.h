@@ -46,9 +46,10 @@ enum ObjKind : uint8_t { KIND_CELL = 0, KIND_MACRO = 1, KIND_FILLER = 2 };
 constexpr double kEps = 1.0e-15;  // global.h:39
 constexpr int kBlock = 256;
 constexpr int kFinish = 1024;  // single-block kernels that sum per-block partials
-constexpr int kSmallNetMaxDegree = 64;  // nets above this get a wave each; the rest are packed 64 pins per wave
+constexpr int kNetLaneMax = 32;  // nets above this get a wave each; the rest a power-of-two lane group each
 constexpr int kBigObjectBins = 16;      // objects that may cover more bins get a block each
-constexpr int kXcds = 8;                // MI355X: workgroups go to the 8 XCDs round-robin
+constexpr int kXcds = 8;
+constexpr int kEvPerEval = 8;  // phase events recorded per evaluation                // MI355X: workgroups go to the 8 XCDs round-robin
 
 // Renumbers blocks so that the blocks of each XCD cover one contiguous
 // 1/8 of the grid (block b runs on XCD b % 8).  Work ranges that are
@@ -134,7 +135,15 @@ struct Ctx {
   std::string err;
   int device = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[6] = {};
+  // second stream: the wiring kernels of an evaluation run on it, beside the
+  // Poisson solve and the field gather (fork after the density finalise,
+  // join before the combine)
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // per-evaluation phase events: 0 start, 1 scatter end, 2 finalise end,
+  // 3 Poisson end, 4 field gather end, 5 combine end (stream), 6 wiring
+  // start, 7 wiring end (stream2)
+  hipEvent_t ev[kEvPerEval] = {};
   double last_timings[5] = {};
   // event pool for per-phase timing inside the optimiser loop (6 per evaluation)
   std::vector<hipEvent_t> prof_events;
@@ -155,7 +164,7 @@ struct Ctx {
   std::vector<float> h_w, h_h;
   std::vector<uint8_t> h_kind;
   std::vector<int32_t> h_stage_idx;  // objects of the current stage, in stage order
-  std::vector<int32_t> h_grad_slot;  // pin_grad slot of each design pin (-1: terminal pin)
+  std::vector<int32_t> h_grad_lane;  // pin_grad entry of each design pin (-1: terminal pin)
   float core_llx = 0, core_lly = 0, core_urx = 0, core_ury = 0, total_row_area = 0;
   float bsx = 0, bsy = 0;
   float std_area = 0, macro_area = 0, filler_w = 0, filler_h = 0;
@@ -178,12 +187,20 @@ struct Ctx {
   float *pin_dx = nullptr, *pin_dy = nullptr;
   int32_t *large_nets = nullptr;
   int num_large = 0;
-  int32_t *group_begin = nullptr, *group_end = nullptr;  // pin ranges of the 64-pin net groups
-  int num_groups = 0;
+  // nets of 2..kNetLaneMax pins by degree class c (D = 2 << c lanes per net):
+  // first stored net, net count, first 64-lane group (wl_gstart[5] = all)
+  int wl_first[5] = {0, 0, 0, 0, 0}, wl_count[5] = {0, 0, 0, 0, 0}, wl_gstart[6] = {0, 0, 0, 0, 0, 0};
+  int wl_groups = 0;
+  // their pins in lane order: object (-1 = padding), offset
+  int32_t *lane_obj = nullptr;
+  float2 *lane_off = nullptr;
   int32_t *pin_net = nullptr;  // net of each pin
-  int32_t *node_pin_start = nullptr;  // [N+1] runs of pin_grad per node
-  int32_t *pin_slot = nullptr;        // [P] pin_grad slot of each device pin, -1 for terminal pins
-  float2 *pin_grad = nullptr;         // [node pins] WA gradients, node by node in modulePins order
+  int32_t *node_pin_start = nullptr;  // [N+1] runs of node_pin_lane per node
+  int32_t *node_pin_lane = nullptr;   // [node pins] pin_grad entry of each node pin, modulePins order
+  float2 *pin_grad = nullptr;         // WA gradient of every pin, in the wiring kernels' lane order
+  int64_t grad_lanes = 0;             // written entries of pin_grad: lane groups, then the pins of large nets;
+                                      // entry grad_lanes stays zero (pins of one-pin nets)
+  int32_t large_lane_off = 0;         // pin_grad entry of large-net pin p = p + large_lane_off
   // bins
   float *bllx = nullptr, *burx = nullptr, *blly = nullptr, *bury = nullptr;
   unsigned long long *node_q = nullptr, *filler_q = nullptr;
@@ -226,11 +243,16 @@ void launch_scatter(Ctx &c);
 void launch_finalize_density(Ctx &c);
 void launch_gather_combine(Ctx &c, bool recompute_gather, bool lipschitz, const float2 *ref_now,
                            const float2 *ref_prev, const float2 *grad_prev);
+void launch_field_gather(Ctx &c);
+void launch_node_wl(Ctx &c, hipStream_t st);
+void launch_combine(Ctx &c, bool recompute, bool lipschitz, const float2 *ref_now, const float2 *ref_prev,
+                    const float2 *grad_prev);
 bool tile_sort_create(Ctx &c, std::string &err);
 void tile_sort_destroy(Ctx &c);
 // ---- wirelength.hip ----
-void launch_wirelength(Ctx &c, bool gradient);
-void launch_reduce_wirelength(Ctx &c, int lambda_update);
+// st: the stream to launch on (nullptr: the context's stream)
+void launch_wirelength(Ctx &c, bool gradient, hipStream_t st = nullptr);
+void launch_reduce_wirelength(Ctx &c, int lambda_update, hipStream_t st = nullptr);
 // ---- poisson.hip ----
 bool poisson_create(Ctx &c, std::string &err);
 void poisson_destroy(Ctx &c);

@@ -21,7 +21,8 @@ namespace ep {
 // Pin positions: centre of the pin's object or terminal (one packed 8-byte
 // read) plus the pin offset (objects.cpp:454-459).
 struct PinSrc {
-  const int32_t *net_start, *pin_obj, *slot;
+  const int32_t *net_start, *pin_obj;
+  int32_t lane_off;  // pin_grad entry of pin p is p + lane_off
   const float *pdx, *pdy;
   const float2 *pos;
   __device__ __forceinline__ float2 at(int p) const {
@@ -74,20 +75,39 @@ __device__ __forceinline__ double net_value(float xmax, float xmin, float ig, fl
   return ((double)xmax + (double)(np / dp)) - ((double)xmin + (double)(nm / dm));
 }
 
-__device__ __forceinline__ float wave_max(float v) {
-  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-  return v;
-}
-__device__ __forceinline__ float wave_min(float v) {
-  for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, 64));
-  return v;
-}
-__device__ __forceinline__ float wave_fsum(float v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+// DPP lane moves (no LDS): row_shr:k reads lane i-k of the same 16-lane
+// row, row_bcast:15 / row_bcast:31 pass lane 15 / 31 on to the following
+// rows.  Lanes whose source falls outside get `old`.
+constexpr int kDppRowShr = 0x110, kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
+
+template <int CTRL, int ROWMASK = 0xF>
+__device__ __forceinline__ float dppf(float old, float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), CTRL, ROWMASK, 0xF, false));
 }
 
-// One wave per net of degree > 64, grid-stride over nets.
+// Whole-wave reduction: scan inside each row, then the row totals through
+// lanes 15 and 31; lane 63 holds the total, read as a wave-uniform value.
+template <class Op>
+__device__ __forceinline__ float wave_reduce_dpp(float v, float id, Op op) {
+  v = op(v, dppf<kDppRowShr + 1>(id, v));
+  v = op(v, dppf<kDppRowShr + 2>(id, v));
+  v = op(v, dppf<kDppRowShr + 4>(id, v));
+  v = op(v, dppf<kDppRowShr + 8>(id, v));
+  v = op(v, dppf<kDppRowBcast15, 0xA>(id, v));
+  v = op(v, dppf<kDppRowBcast31, 0xC>(id, v));
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+__device__ __forceinline__ float wave_max(float v) {
+  return wave_reduce_dpp(v, -INFINITY, [](float a, float b) { return fmaxf(a, b); });
+}
+__device__ __forceinline__ float wave_min(float v) {
+  return wave_reduce_dpp(v, INFINITY, [](float a, float b) { return fminf(a, b); });
+}
+__device__ __forceinline__ float wave_fsum(float v) {
+  return wave_reduce_dpp(v, 0.f, [](float a, float b) { return a + b; });
+}
+
+// One wave per net of more than kNetLaneMax pins, grid-stride over nets.
 template <int MODEL, int EXPM, int GRAD>
 __global__ void __launch_bounds__(kBlock) wl_large_kernel(int num, const int32_t *__restrict__ nets,
                                                            PinSrc src, const DevScalars *sc,
@@ -151,8 +171,7 @@ __global__ void __launch_bounds__(kBlock) wl_large_kernel(int num, const int32_t
       pin_terms<MODEL, EXPM>(py, ymax, ymin, igy, fp, fm);
       float gx = pin_grad_axis<MODEL>(px, xmax, xmin, igx, ep, em, a.dpx, a.npx, a.dmx, a.nmx);
       float gy = pin_grad_axis<MODEL>(py, ymax, ymin, igy, fp, fm, a.dpy, a.npy, a.dmy, a.nmy);
-      const int sl = src.slot[p];
-      if (sl >= 0) pin_grad[sl] = make_float2(gx, gy);
+      pin_grad[p + src.lane_off] = make_float2(gx, gy);
     }
   }
   hp = block_sum(hp);
@@ -163,87 +182,108 @@ __global__ void __launch_bounds__(kBlock) wl_large_kernel(int num, const int32_t
   }
 }
 
-// Segmented reductions inside a wave over runs of lanes holding the same
-// net.  same[k] says lane+2^k holds the same net as this lane.
-template <class Op>
-__device__ __forceinline__ float seg_reduce(float v, const bool *same, Op op) {
-#pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    float o = __shfl_down(v, 1 << k, 64);
-    if (same[k]) v = op(v, o);
-  }
+// Butterfly all-reduce inside aligned groups of D lanes (D = 2..32): lane
+// xor 1 and xor 2 by quad permutes, xor 4 by the half-row mirror (the other
+// quad of an 8-lane half holds that quad's result), xor 8 by the row mirror,
+// xor 16 by a lane permute.  Every lane of a group ends with the same value.
+constexpr int kDppQuadXor1 = 0xB1, kDppQuadXor2 = 0x4E, kDppRowMirror = 0x140, kDppRowHalfMirror = 0x141;
+
+template <int D, class Op>
+__device__ __forceinline__ float group_allreduce(float v, Op op) {
+  if (D >= 2) v = op(v, dppf<kDppQuadXor1>(v, v));
+  if (D >= 4) v = op(v, dppf<kDppQuadXor2>(v, v));
+  if (D >= 8) v = op(v, dppf<kDppRowHalfMirror>(v, v));
+  if (D >= 16) v = op(v, dppf<kDppRowMirror>(v, v));
+  if (D >= 32) v = op(v, __shfl_xor(v, 16, 64));
   return v;
 }
 
-// One wave per group of consecutive nets of total degree <= 64, one pin per
-// lane (coalesced pin loads, no divergence on net degree).  Bound pins,
-// HPWL, the WA/LSE sums and the per-pin gradient come from segmented
-// shuffle reductions over each net's lanes; the net's head lane holds the
-// result and broadcasts it.  Each wave takes a contiguous run of groups and
-// the blocks are numbered XCD-contiguously, so with nets stored by their
-// lowest node one XCD reads the centres of a narrow band of objects.
+// Nets of 2..kNetLaneMax pins, grouped by degree class: a net of degree d in
+// (D/2, D] takes D aligned lanes (pins in the first d, padding after), 64/D
+// nets per 64-lane group.  The pins are stored in that lane order (object
+// index, offset, gradient slot; object -1 for padding), so a group reads its
+// pins with coalesced loads and one gather of centres.  Net-wide bound pins
+// and the WA / LSE sums are butterfly all-reduces inside the D lanes (no
+// segment flags, no broadcast); the net's first lane adds the net's HPWL and
+// value.  Gradients are written in lane order (coalesced); the node sums
+// read them back through node_pin_lane.
+struct WlClasses {
+  int group_start[6];  // first 64-lane group of class c (D = 2 << c); group_start[5] = all groups
+};
+
+struct LanePins {
+  const int32_t *obj;
+  const float2 *off;
+  const float2 *pos;
+};
+
+template <int D, int MODEL, int EXPM, int GRAD>
+__device__ __forceinline__ void wl_group_d(int g, const LanePins &lp, float igx, float igy,
+                                           float2 *__restrict__ pin_grad, double &hp, double &wv) {
+  const int lane = threadIdx.x & 63;
+  const int idx = g * 64 + lane;
+  const int obj = lp.obj[idx];
+  const bool act = obj >= 0;
+  float px = 0.f, py = 0.f;
+  if (act) {
+    const float2 c = lp.pos[obj], o = lp.off[idx];
+    px = c.x + o.x;
+    py = c.y + o.y;
+  }
+  auto fmax_ = [](float a, float b) { return fmaxf(a, b); };
+  auto fmin_ = [](float a, float b) { return fminf(a, b); };
+  auto fadd_ = [](float a, float b) { return a + b; };
+  const float xmax = group_allreduce<D>(act ? px : -INFINITY, fmax_);
+  const float xmin = group_allreduce<D>(act ? px : INFINITY, fmin_);
+  const float ymax = group_allreduce<D>(act ? py : -INFINITY, fmax_);
+  const float ymin = group_allreduce<D>(act ? py : INFINITY, fmin_);
+  const bool head = act && (lane % D) == 0;
+  if (head) hp += ((double)xmax - (double)xmin) + ((double)ymax - (double)ymin);
+  if (!GRAD) return;
+  float ep = 0.f, em = 0.f, fp = 0.f, fm = 0.f;
+  if (act) {
+    pin_terms<MODEL, EXPM>(px, xmax, xmin, igx, ep, em);
+    pin_terms<MODEL, EXPM>(py, ymax, ymin, igy, fp, fm);
+  }
+  const float dpx = group_allreduce<D>(ep, fadd_);
+  const float npx = group_allreduce<D>((px - xmax) * ep, fadd_);
+  const float dmx = group_allreduce<D>(em, fadd_);
+  const float nmx = group_allreduce<D>((px - xmin) * em, fadd_);
+  const float dpy = group_allreduce<D>(fp, fadd_);
+  const float npy = group_allreduce<D>((py - ymax) * fp, fadd_);
+  const float dmy = group_allreduce<D>(fm, fadd_);
+  const float nmy = group_allreduce<D>((py - ymin) * fm, fadd_);
+  if (head)
+    wv += net_value<MODEL>(xmax, xmin, igx, dpx, npx, dmx, nmx) + net_value<MODEL>(ymax, ymin, igy, dpy, npy, dmy, nmy);
+  if (act) {
+    const float gx = pin_grad_axis<MODEL>(px, xmax, xmin, igx, ep, em, dpx, npx, dmx, nmx);
+    const float gy = pin_grad_axis<MODEL>(py, ymax, ymin, igy, fp, fm, dpy, npy, dmy, nmy);
+    pin_grad[idx] = make_float2(gx, gy);
+  }
+}
+
+// Each wave takes a contiguous run of 64-lane groups (XCD-contiguous block
+// numbering); the class of a group is wave-uniform.
 template <int MODEL, int EXPM, int GRAD>
-__global__ void __launch_bounds__(kBlock) wl_group_kernel(int num_groups, const int32_t *__restrict__ gbeg,
-                                                           const int32_t *__restrict__ gend,
-                                                           const int32_t *__restrict__ pin_net, PinSrc src,
-                                                           const DevScalars *sc, float2 *__restrict__ pin_grad,
-                                                           double *partials) {
+__global__ void __launch_bounds__(kBlock) wl_group_kernel(WlClasses cls, LanePins lp, const DevScalars *sc,
+                                                           float2 *__restrict__ pin_grad, double *partials) {
   if (sc->done) return;
   const float igx = (float)sc->inv_gamma_x, igy = (float)sc->inv_gamma_y;
-  const int lane = threadIdx.x & 63;
   const int wpb = blockDim.x >> 6;
   const int waves = gridDim.x * wpb, wid = xcd_block() * wpb + (threadIdx.x >> 6);
-  const int per = (num_groups + waves - 1) / waves;
-  const int gi0 = wid * per, gi1 = min(num_groups, gi0 + per);
+  const int total = cls.group_start[5];
+  const int per = (total + waves - 1) / waves;
+  const int g0 = wid * per, g1 = min(total, g0 + per);
   double hp = 0, wv = 0;
-  for (int gi = gi0; gi < gi1; ++gi) {
-    const int p0 = gbeg[gi], p1 = gend[gi];
-    const int p = p0 + lane;
-    const bool act = p < p1;
-    int net = act ? pin_net[p] : -1 - lane;  // inactive lanes: unique segments
-    float px = 0.f, py = 0.f;
-    if (act) {
-      const float2 q = src.at(p);
-      px = q.x;
-      py = q.y;
-    }
-    bool same[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      int on = __shfl_down(net, 1 << k, 64);
-      same[k] = (lane + (1 << k) < 64) && on == net;
-    }
-    int head = act ? src.net_start[net] - p0 : lane;
-    auto fmax_ = [](float a, float b) { return fmaxf(a, b); };
-    auto fmin_ = [](float a, float b) { return fminf(a, b); };
-    auto fadd_ = [](float a, float b) { return a + b; };
-    float xmax = __shfl(seg_reduce(px, same, fmax_), head, 64);
-    float xmin = __shfl(seg_reduce(px, same, fmin_), head, 64);
-    float ymax = __shfl(seg_reduce(py, same, fmax_), head, 64);
-    float ymin = __shfl(seg_reduce(py, same, fmin_), head, 64);
-    const bool is_head = act && head == lane;
-    if (is_head) hp += ((double)xmax - (double)xmin) + ((double)ymax - (double)ymin);
-    if (!GRAD) continue;
-    float ep = 0.f, em = 0.f, fp = 0.f, fm = 0.f;
-    if (act) {
-      pin_terms<MODEL, EXPM>(px, xmax, xmin, igx, ep, em);
-      pin_terms<MODEL, EXPM>(py, ymax, ymin, igy, fp, fm);
-    }
-    float dpx = __shfl(seg_reduce(ep, same, fadd_), head, 64);
-    float npx = __shfl(seg_reduce((px - xmax) * ep, same, fadd_), head, 64);
-    float dmx = __shfl(seg_reduce(em, same, fadd_), head, 64);
-    float nmx = __shfl(seg_reduce((px - xmin) * em, same, fadd_), head, 64);
-    float dpy = __shfl(seg_reduce(fp, same, fadd_), head, 64);
-    float npy = __shfl(seg_reduce((py - ymax) * fp, same, fadd_), head, 64);
-    float dmy = __shfl(seg_reduce(fm, same, fadd_), head, 64);
-    float nmy = __shfl(seg_reduce((py - ymin) * fm, same, fadd_), head, 64);
-    if (is_head)
-      wv += net_value<MODEL>(xmax, xmin, igx, dpx, npx, dmx, nmx) + net_value<MODEL>(ymax, ymin, igy, dpy, npy, dmy, nmy);
-    if (act) {
-      float gx = pin_grad_axis<MODEL>(px, xmax, xmin, igx, ep, em, dpx, npx, dmx, nmx);
-      float gy = pin_grad_axis<MODEL>(py, ymax, ymin, igy, fp, fm, dpy, npy, dmy, nmy);
-      const int sl = src.slot[p];
-      if (sl >= 0) pin_grad[sl] = make_float2(gx, gy);
+  for (int g = g0; g < g1; ++g) {
+    int c = 0;
+    while (c < 4 && g >= cls.group_start[c + 1]) ++c;
+    switch (c) {
+      case 0: wl_group_d<2, MODEL, EXPM, GRAD>(g, lp, igx, igy, pin_grad, hp, wv); break;
+      case 1: wl_group_d<4, MODEL, EXPM, GRAD>(g, lp, igx, igy, pin_grad, hp, wv); break;
+      case 2: wl_group_d<8, MODEL, EXPM, GRAD>(g, lp, igx, igy, pin_grad, hp, wv); break;
+      case 3: wl_group_d<16, MODEL, EXPM, GRAD>(g, lp, igx, igy, pin_grad, hp, wv); break;
+      default: wl_group_d<32, MODEL, EXPM, GRAD>(g, lp, igx, igy, pin_grad, hp, wv); break;
     }
   }
   hp = block_sum(hp);
@@ -279,9 +319,11 @@ __global__ void __launch_bounds__(kFinish) reduce_wl_kernel(int nslots, double *
 }
 
 static int g_small_blocks(const Ctx &c) {
-  int wpb = kBlock / 64;
-  int b = (c.num_groups + wpb - 1) / wpb;
-  b = b < 1 ? 1 : (b > 8192 ? 8192 : b);
+  const int wpb = kBlock / 64;
+  const int groups = c.wl_groups;
+  int b = (groups + wpb - 1) / wpb;
+  // 1024 blocks leave room for the transforms running beside this kernel
+  b = b < 1 ? 1 : (b > 1024 ? 1024 : b);
   return (b + kXcds - 1) / kXcds * kXcds;
 }
 static int g_large_blocks(const Ctx &c) {
@@ -291,34 +333,39 @@ static int g_large_blocks(const Ctx &c) {
 }
 
 template <int MODEL, int EXPM, int GRAD>
-static void launch_wl_t(Ctx &c) {
-  PinSrc src{c.net_start, c.pin_obj, c.pin_slot, c.pin_dx, c.pin_dy, c.pos};
+static void launch_wl_t(Ctx &c, hipStream_t st) {
+  PinSrc src{c.net_start, c.pin_obj, c.large_lane_off, c.pin_dx, c.pin_dy, c.pos};
   int bs = g_small_blocks(c);
-  hipLaunchKernelGGL((wl_group_kernel<MODEL, EXPM, GRAD>), dim3(bs), dim3(kBlock), 0, c.stream, c.num_groups,
-                     c.group_begin, c.group_end, c.pin_net, src, c.sc, c.pin_grad, c.partials);
+  WlClasses cls;
+  for (int k = 0; k < 6; ++k) cls.group_start[k] = c.wl_gstart[k];
+  LanePins lp{c.lane_obj, c.lane_off, c.pos};
+  hipLaunchKernelGGL((wl_group_kernel<MODEL, EXPM, GRAD>), dim3(bs), dim3(kBlock), 0, st, cls, lp, c.sc,
+                     c.pin_grad, c.partials);
   if (c.num_large) {
     hipLaunchKernelGGL((wl_large_kernel<MODEL, EXPM, GRAD>), dim3(g_large_blocks(c)), dim3(kBlock), 0,
-                       c.stream, c.num_large, c.large_nets, src, c.sc, c.pin_grad, c.partials, bs);
+                       st, c.num_large, c.large_nets, src, c.sc, c.pin_grad, c.partials, bs);
   }
 }
 
-void launch_wirelength(Ctx &c, bool gradient) {
+void launch_wirelength(Ctx &c, bool gradient, hipStream_t st) {
+  if (!st) st = c.stream;
   if (!gradient) {
-    launch_wl_t<0, 0, 0>(c);
+    launch_wl_t<0, 0, 0>(c, st);
     return;
   }
   if (c.wl_model == 0) {
-    if (c.exp_mode == 0) launch_wl_t<0, 0, 1>(c);
-    else launch_wl_t<0, 1, 1>(c);
+    if (c.exp_mode == 0) launch_wl_t<0, 0, 1>(c, st);
+    else launch_wl_t<0, 1, 1>(c, st);
   } else {
-    if (c.exp_mode == 0) launch_wl_t<1, 0, 1>(c);
-    else launch_wl_t<1, 1, 1>(c);
+    if (c.exp_mode == 0) launch_wl_t<1, 0, 1>(c, st);
+    else launch_wl_t<1, 1, 1>(c, st);
   }
 }
 
-void launch_reduce_wirelength(Ctx &c, int lambda_update) {
+void launch_reduce_wirelength(Ctx &c, int lambda_update, hipStream_t st) {
+  if (!st) st = c.stream;
   int n = g_small_blocks(c) + (c.num_large ? g_large_blocks(c) : 0);
-  hipLaunchKernelGGL(reduce_wl_kernel, dim3(1), dim3(kFinish), 0, c.stream, n, c.partials, c.sc, lambda_update);
+  hipLaunchKernelGGL(reduce_wl_kernel, dim3(1), dim3(kFinish), 0, st, n, c.partials, c.sc, lambda_update);
 }
 
 }  // namespace ep
