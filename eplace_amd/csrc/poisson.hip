@@ -96,8 +96,17 @@ struct PGemm {
 };
 
 constexpr int PBM = 64, PBN = 64, PK = 32;
+constexpr int kGroups = 2;  // k-groups per block: each runs half the k steps, then they add up
+constexpr int kGroupThreads = 256;
+
+// Thread index inside the thread's k-group.
+__device__ __forceinline__ int gtid() { return threadIdx.x & (kGroupThreads - 1); }
 constexpr int kLd32 = PK + 4;  // FP32 LDS row stride (floats), rows 16-byte aligned
-constexpr int kLd16 = PK + 8;  // bf16 LDS row stride (elements), rows 16-byte aligned
+// bf16 LDS tiles: 64-byte rows (32 k) of four 16-byte chunks, chunk c of row
+// r stored at c ^ ((r >> 1 ^ r >> 3) & 3).  The stores (whole rows) and the
+// MFMA operand reads (16 consecutive rows, one chunk) then touch distinct
+// 16-byte bank groups per 8 lanes with 32 banks and per 16 lanes with 64.
+__device__ __forceinline__ int swz16(int r, int c) { return 8 * (c ^ ((r >> 1 ^ r >> 3) & 3)); }
 
 // Per-thread share of one k step: row tid / 4 of the 64-row A and B tiles,
 // eight consecutive k at 8 (tid % 4).
@@ -112,12 +121,12 @@ struct Smem32 {
   float A[2][PBM][kLd32], B[2][PBN][kLd32];
 };
 struct Smem16 {
-  __bf16 Ah[2][PBM][kLd16], Al[2][PBM][kLd16], Bh[2][PBN][kLd16], Bl[2][PBN][kLd16];
+  __bf16 Ah[2][PBM][PK], Al[2][PBM][PK], Bh[2][PBN][PK], Bl[2][PBN][PK];
 };
 
 __device__ __forceinline__ void load_b(const PGemm &p, const float *__restrict__ B, float sb, float sub, int col0,
                                        int k0, float (&v)[8]) {
-  const int r = threadIdx.x >> 2, k = k0 + 8 * (threadIdx.x & 3);
+  const int r = gtid() >> 2, k = k0 + 8 * (gtid() & 3);
   const float *row = B + (size_t)(col0 + r) * p.ldb;
   const float4 a0 = *reinterpret_cast<const float4 *>(row + k);
   const float4 a1 = *reinterpret_cast<const float4 *>(row + k + 4);
@@ -144,7 +153,7 @@ __device__ __forceinline__ void load_b(const PGemm &p, const float *__restrict__
 
 __device__ __forceinline__ void load_stage(const PGemm &p, int z, const float *B, float sb, float sub, int row0,
                                            int col0, int k0, Stage32 &st) {
-  const int r = threadIdx.x >> 2, k = k0 + 8 * (threadIdx.x & 3);
+  const int r = gtid() >> 2, k = k0 + 8 * (gtid() & 3);
   const float *a = p.Af[z] + (size_t)(row0 + r) * p.lda + k;
   const float4 a0 = *reinterpret_cast<const float4 *>(a);
   const float4 a1 = *reinterpret_cast<const float4 *>(a + 4);
@@ -155,7 +164,7 @@ __device__ __forceinline__ void load_stage(const PGemm &p, int z, const float *B
 
 __device__ __forceinline__ void load_stage(const PGemm &p, int z, const float *B, float sb, float sub, int row0,
                                            int col0, int k0, Stage16 &st) {
-  const int r = threadIdx.x >> 2, k = k0 + 8 * (threadIdx.x & 3);
+  const int r = gtid() >> 2, k = k0 + 8 * (gtid() & 3);
   const size_t off = (size_t)(row0 + r) * p.lda + k;
   st.ah = *reinterpret_cast<const bf16x8 *>(p.Ah[z] + off);
   st.al = *reinterpret_cast<const bf16x8 *>(p.Al[z] + off);
@@ -163,7 +172,7 @@ __device__ __forceinline__ void load_stage(const PGemm &p, int z, const float *B
 }
 
 __device__ __forceinline__ void store_stage(Smem32 &s, int buf, const Stage32 &st) {
-  const int r = threadIdx.x >> 2, k = 8 * (threadIdx.x & 3);
+  const int r = gtid() >> 2, k = 8 * (gtid() & 3);
   *reinterpret_cast<float4 *>(&s.A[buf][r][k]) = make_float4(st.a[0], st.a[1], st.a[2], st.a[3]);
   *reinterpret_cast<float4 *>(&s.A[buf][r][k + 4]) = make_float4(st.a[4], st.a[5], st.a[6], st.a[7]);
   *reinterpret_cast<float4 *>(&s.B[buf][r][k]) = make_float4(st.b[0], st.b[1], st.b[2], st.b[3]);
@@ -171,7 +180,7 @@ __device__ __forceinline__ void store_stage(Smem32 &s, int buf, const Stage32 &s
 }
 
 __device__ __forceinline__ void store_stage(Smem16 &s, int buf, const Stage16 &st) {
-  const int r = threadIdx.x >> 2, k = 8 * (threadIdx.x & 3);
+  const int r = gtid() >> 2, k = swz16(r, gtid() & 3);
   *reinterpret_cast<bf16x8 *>(&s.Ah[buf][r][k]) = st.ah;
   *reinterpret_cast<bf16x8 *>(&s.Al[buf][r][k]) = st.al;
   bf16x8 h, l;
@@ -188,7 +197,7 @@ __device__ __forceinline__ void store_stage(Smem16 &s, int buf, const Stage16 &s
 // FP32: lane group g = lane / 16 takes k = 8 g + s in sub-step s for both
 // operands, so every 16x16x4 sub-step pairs the same k values.
 __device__ __forceinline__ void mma(const Smem32 &s, int buf, floatx4 (&acc)[2][2]) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = gtid() >> 6;
   const int kq = 8 * (lane >> 4);
   float a[2][8], b[2][8];
 #pragma unroll
@@ -217,20 +226,19 @@ __device__ __forceinline__ void mma(const Smem32 &s, int buf, floatx4 (&acc)[2][
 // split-BF16: lane group g supplies the eight k values 8 g .. 8 g + 7 of its
 // row (A) and column (B); both operands use the same slots.
 __device__ __forceinline__ void mma(const Smem16 &s, int buf, floatx4 (&acc)[2][2]) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int kq = 8 * (lane >> 4);
+  const int lane = threadIdx.x & 63, wave = gtid() >> 6;
   bf16x8 ah[2], al[2], bh[2], bl[2];
 #pragma unroll
   for (int mi = 0; mi < 2; ++mi) {
     const int r = (wave >> 1) * 32 + mi * 16 + (lane & 15);
-    ah[mi] = *reinterpret_cast<const bf16x8 *>(&s.Ah[buf][r][kq]);
-    al[mi] = *reinterpret_cast<const bf16x8 *>(&s.Al[buf][r][kq]);
+    ah[mi] = *reinterpret_cast<const bf16x8 *>(&s.Ah[buf][r][swz16(r, lane >> 4)]);
+    al[mi] = *reinterpret_cast<const bf16x8 *>(&s.Al[buf][r][swz16(r, lane >> 4)]);
   }
 #pragma unroll
   for (int ni = 0; ni < 2; ++ni) {
     const int r = (wave & 1) * 32 + ni * 16 + (lane & 15);
-    bh[ni] = *reinterpret_cast<const bf16x8 *>(&s.Bh[buf][r][kq]);
-    bl[ni] = *reinterpret_cast<const bf16x8 *>(&s.Bl[buf][r][kq]);
+    bh[ni] = *reinterpret_cast<const bf16x8 *>(&s.Bh[buf][r][swz16(r, lane >> 4)]);
+    bl[ni] = *reinterpret_cast<const bf16x8 *>(&s.Bl[buf][r][swz16(r, lane >> 4)]);
   }
 #pragma unroll
   for (int mi = 0; mi < 2; ++mi)
@@ -242,39 +250,73 @@ __device__ __forceinline__ void mma(const Smem16 &s, int buf, floatx4 (&acc)[2][
     }
 }
 
-// acc += A[rows, k_begin..k_end) B[cols, k_begin..k_end)^T: LDS double
-// buffer, the next k step's global loads in flight during the multiply,
-// one barrier per step.
+// acc += A[rows, k_begin..k_end) B[cols, k_begin..k_end)^T, the k steps
+// split between the block's k-groups (group g takes steps g, g + kGroups,
+// ...).  Each group has its own LDS double buffer with the next step's
+// global loads in flight during the multiply; every group passes the same
+// number of block barriers (one per round), also when it has no step left.
 template <class ST, class SM>
 __device__ __forceinline__ void run_k(const PGemm &p, int z, const float *B, float sb, float sub, int row0,
                                       int col0, int k_begin, int k_end, SM &s, floatx4 (&acc)[2][2]) {
+  const int grp = threadIdx.x / kGroupThreads;
+  const int steps = (k_end - k_begin + PK - 1) / PK;
+  const int rounds = (steps + kGroups - 1) / kGroups;
+  auto kstep = [&](int r) { return k_begin + (r * kGroups + grp) * PK; };
   ST st;
-  load_stage(p, z, B, sb, sub, row0, col0, k_begin, st);
+  if (kstep(0) < k_end) load_stage(p, z, B, sb, sub, row0, col0, kstep(0), st);
   __syncthreads();  // the previous pass may still read buffer 0
-  store_stage(s, 0, st);
+  if (kstep(0) < k_end) store_stage(s, 0, st);
   __syncthreads();
   int buf = 0;
-  for (int k0 = k_begin; k0 < k_end; k0 += PK) {
-    const bool next = k0 + PK < k_end;
-    if (next) load_stage(p, z, B, sb, sub, row0, col0, k0 + PK, st);
-    mma(s, buf, acc);
+  for (int r = 0; r < rounds; ++r) {
+    const bool have = kstep(r) < k_end, next = kstep(r + 1) < k_end;
+    if (next) load_stage(p, z, B, sb, sub, row0, col0, kstep(r + 1), st);
+    if (have) mma(s, buf, acc);
     if (next) store_stage(s, buf ^ 1, st);
     __syncthreads();
     buf ^= 1;
   }
 }
 
+// Adds the other k-groups' accumulators into group 0's through LDS (the
+// groups' stage buffers are free after the last round's barrier).
+template <class SM>
+__device__ __forceinline__ void reduce_groups(SM (&s)[kGroups], floatx4 (&acc)[2][2]) {
+  float *red = reinterpret_cast<float *>(&s[0]);
+  const int grp = threadIdx.x / kGroupThreads, t = gtid();
+  for (int g = 1; g < kGroups; ++g) {
+    if (grp == g)
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) red[((i * 2 + j) * 4 + r) * kGroupThreads + t] = acc[i][j][r];
+    __syncthreads();
+    if (grp == 0)
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc[i][j][r] += red[((i * 2 + j) * 4 + r) * kGroupThreads + t];
+    __syncthreads();
+  }
+}
+
 template <int BF16>
-__global__ void __launch_bounds__(256) pgemm_kernel(PGemm p) {
+__global__ void __launch_bounds__(kGroups * kGroupThreads) pgemm_kernel(PGemm p) {
   typedef typename std::conditional<BF16 != 0, Stage16, Stage32>::type ST;
   typedef typename std::conditional<BF16 != 0, Smem16, Smem32>::type SM;
-  __shared__ __attribute__((aligned(16))) SM s;
+  static_assert(sizeof(SM) >= 16 * kGroupThreads * sizeof(float), "reduction buffer");
+  __shared__ __attribute__((aligned(16))) SM s[kGroups];
   const int z = blockIdx.z;
   const float *__restrict__ B = p.B[z];
   float *__restrict__ C = p.C[z];
   const float sb = p.sb[z];
   const float sub = p.sub ? (float)*p.sub : 0.f;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int grp = threadIdx.x / kGroupThreads;
+  const int lane = threadIdx.x & 63, wave = gtid() >> 6;
   const int row0 = blockIdx.y * PBM, col0 = blockIdx.x * PBN;
   const int roww = row0 + (wave >> 1) * 32, colw = col0 + (wave & 1) * 32;
   floatx4 acc[2][2], odd[2][2];
@@ -286,11 +328,15 @@ __global__ void __launch_bounds__(256) pgemm_kernel(PGemm p) {
       odd[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
     }
   if (p.out_fold) {
-    run_k<ST>(p, z, B, sb, sub, row0, col0, 0, p.khalf, s, acc);
-    run_k<ST>(p, z, B, sb, sub, row0, col0, p.khalf, p.K, s, odd);
+    run_k<ST>(p, z, B, sb, sub, row0, col0, 0, p.khalf, s[grp], acc);
+    run_k<ST>(p, z, B, sb, sub, row0, col0, p.khalf, p.K, s[grp], odd);
+    reduce_groups(s, acc);
+    reduce_groups(s, odd);
   } else {
-    run_k<ST>(p, z, B, sb, sub, row0, col0, 0, p.K, s, acc);
+    run_k<ST>(p, z, B, sb, sub, row0, col0, 0, p.K, s[grp], acc);
+    reduce_groups(s, acc);
   }
+  if (grp != 0) return;
   const float se = p.se[z], so = p.so[z];
 #pragma unroll
   for (int mi = 0; mi < 2; ++mi)
@@ -318,8 +364,9 @@ __global__ void __launch_bounds__(256) pgemm_kernel(PGemm p) {
 
 static void gemm(Ctx &c, const PGemm &p, int batch) {
   dim3 grid(p.N / PBN, p.M / PBM, batch);
-  if (c.poisson->mode == 1) hipLaunchKernelGGL(pgemm_kernel<1>, grid, dim3(256), 0, c.stream, p);
-  else hipLaunchKernelGGL(pgemm_kernel<0>, grid, dim3(256), 0, c.stream, p);
+  const dim3 block(kGroups * kGroupThreads);
+  if (c.poisson->mode == 1) hipLaunchKernelGGL(pgemm_kernel<1>, grid, block, 0, c.stream, p);
+  else hipLaunchKernelGGL(pgemm_kernel<0>, grid, block, 0, c.stream, p);
 }
 
 static int round_up(int v, int m) { return (v + m - 1) / m * m; }
