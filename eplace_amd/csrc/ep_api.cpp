@@ -158,6 +158,7 @@ int create(Ctx &c, const ep_design *d, const ep_params *p) {
   EP_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
   EP_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
   for (auto &e : c.ev) EP_HIP(hipEventCreate(&e));
+  EP_HIP(hipHostMalloc((void **)&c.h_sc, sizeof(DevScalars), hipHostMallocDefault));
   c.td = p->target_density;
   c.wl_model = p->wirelength_model;
   c.exp_mode = p->exp_mode;
@@ -503,6 +504,7 @@ void destroy(Ctx &c) {
   for (auto &e : c.ev)
     if (e) (void)hipEventDestroy(e);
   for (auto &e : c.prof_events) (void)hipEventDestroy(e);
+  if (c.h_sc) (void)hipHostFree(c.h_sc);
   if (c.ev_fork) (void)hipEventDestroy(c.ev_fork);
   if (c.ev_join) (void)hipEventDestroy(c.ev_join);
   if (c.stream2) {
@@ -571,9 +573,10 @@ int sync(Ctx &c) {
 }
 
 int read_scalars(Ctx &c, DevScalars &h) {
+  EP_HIP(hipMemcpyAsync(c.h_sc, c.sc, sizeof(h), hipMemcpyDeviceToHost, c.stream));
   int rc = sync(c);
   if (rc) return rc;
-  EP_HIP(hipMemcpy(&h, c.sc, sizeof(h), hipMemcpyDeviceToHost));
+  h = *c.h_sc;
   return EP_OK;
 }
 
@@ -714,6 +717,7 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
       launch_positions_to_ref(c, c.cur_ref);
       launch_copy_float2(c, c.cur_grad, c.gtot, c.O);
       int tries = 0;
+      bool stopped = false;
       while (true) {
         launch_nesterov_move_ex(c, c.cur_ref, c.cur_grad, c.cur_main, c.new_main, c.new_ref, nullptr, nullptr);
         evaluate(c, false, true, c.new_ref, c.cur_ref, c.cur_grad, false);
@@ -721,8 +725,19 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
         DevScalars h;
         if ((rc = read_scalars(c, h))) return rc;
         ++tries;
-        if (h.done || h.accepted || tries >= o.max_backtracks) break;
+        // Only the stop check at the end of an iteration sets done, and the
+        // iteration is enqueued without waiting for it: done seen here means
+        // the previous iteration ended the run and this one's kernels
+        // returned without work.
+        if (h.done) {
+          stopped = true;
+          // that evaluation's phase events time kernels that did nothing
+          if (c.profiling && c.prof_used >= (size_t)kEvPerEval) c.prof_used -= kEvPerEval;
+          break;
+        }
+        if (h.accepted || tries >= o.max_backtracks) break;
       }
+      if (stopped) break;
       std::swap(c.last_ref, c.cur_ref);
       std::swap(c.last_grad, c.cur_grad);
       std::swap(c.cur_main, c.new_main);
@@ -731,6 +746,7 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
       launch_reduce_wirelength(c, learned ? 2 : 1);
       launch_done_check_ex(c, k + 1, dlog, cap, 1);
       ++k;
+      continue;
     }
     DevScalars h;
     if ((rc = read_scalars(c, h))) return rc;

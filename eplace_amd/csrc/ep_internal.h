@@ -140,6 +140,9 @@ struct Ctx {
   // join before the combine)
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // pinned host copy of the scalars: a read is one copy queued behind the
+  // stream's work and one wait
+  DevScalars *h_sc = nullptr;
   // per-evaluation phase events: 0 start, 1 scatter end, 2 finalise end,
   // 3 Poisson end, 4 field gather end, 5 combine end (stream), 6 wiring
   // start, 7 wiring end (stream2)
