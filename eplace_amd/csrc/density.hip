@@ -64,14 +64,6 @@ __device__ __forceinline__ double bin_overlap(const Grid &g, int i, int j, const
   return hl * vl;
 }
 
-__global__ void zero_maps_kernel(int64_t n, unsigned long long *a, unsigned long long *b) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    a[i] = 0;
-    b[i] = 0;
-  }
-}
-
 // Addend of eplace.cpp:1116/1123/1127 for one bin, as int64 units of 2^-shift.
 __device__ __forceinline__ unsigned long long fixed_contribution(float s, double ov, double scale) {
   float c = s * (float)ov;
@@ -102,13 +94,19 @@ __device__ __forceinline__ uint32_t tile_key(const float2 p, float w, float h, u
 
 // Pass 1: block b counts the keys of objects [b per, (b+1) per) in LDS and
 // reserves its run inside each key with one atomic on the key's running
-// total (bcount[b][k] = the run's offset inside the key).
+// total (bcount[b][k] = the run's offset inside the key).  The grid also
+// clears the nb-bin fixed-point maps the scatter accumulates into.
 __global__ void __launch_bounds__(kSortThreads) tile_hist_kernel(
     int O, int per, const float2 *__restrict__ pos, const float *__restrict__ w, const float *__restrict__ h,
     const uint8_t *__restrict__ kind, const uint8_t *__restrict__ is_big, Grid g, int tyc, int ntiles, int nkeys,
-    uint32_t *okey, uint32_t *bcount, uint32_t *ktotal) {
+    uint32_t *okey, uint32_t *bcount, uint32_t *ktotal, int64_t nb, unsigned long long *node_q,
+    unsigned long long *filler_q) {
   extern __shared__ uint32_t hist[];
   for (int k = threadIdx.x; k < nkeys; k += blockDim.x) hist[k] = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += (int64_t)gridDim.x * blockDim.x) {
+    node_q[i] = 0;
+    filler_q[i] = 0;
+  }
   __syncthreads();
   const int o0 = blockIdx.x * per, o1 = min(O, o0 + per);
   for (int o = o0 + threadIdx.x; o < o1; o += blockDim.x) {
@@ -277,7 +275,8 @@ __global__ void __launch_bounds__(kBlock) finalize_density_kernel(
 }
 
 // tau = overflow area / node area, mean rho, and the gamma schedule
-// (eplace.cpp:475-500) in the reference's float arithmetic.
+// (eplace.cpp:475-500) in the reference's float arithmetic; counts the
+// evaluation.
 __global__ void __launch_bounds__(kFinish) finish_density_kernel(int nblocks, int nb, Grid g,
                                                                  double node_area_scaled, const double *partials,
                                                                  DevScalars *sc) {
@@ -303,6 +302,7 @@ __global__ void __launch_bounds__(kFinish) finish_density_kernel(int nblocks, in
     }
     sc->inv_gamma_x = cx;
     sc->inv_gamma_y = cy;
+    sc->evals += 1;
   }
 }
 
@@ -507,19 +507,14 @@ __global__ void __launch_bounds__(kFinish) lambda_init_finish_kernel(int nblocks
 // ---------------------------------------------------------------------------
 static int blocks_for(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
 
-void launch_zero_maps(Ctx &c) {
-  int64_t nb = (int64_t)c.nx * c.ny;
-  hipLaunchKernelGGL(zero_maps_kernel, dim3(blocks_for(nb)), dim3(kBlock), 0, c.stream, nb, c.node_q,
-                     c.filler_q);
-}
-
 void launch_scatter(Ctx &c) {
   Grid g = c.grid();
   double scale = ldexp(1.0, c.shift);
   TileSort &t = c.tiles;
   const size_t lds = (size_t)t.nkeys * sizeof(uint32_t);
   hipLaunchKernelGGL(tile_hist_kernel, dim3(t.nblocks), dim3(kSortThreads), lds, c.stream, c.O, t.per, c.pos, c.w,
-                     c.h, c.kind, c.is_big, g, t.tyc, t.ntiles, t.nkeys, t.okey, t.bcount, t.ktotal);
+                     c.h, c.kind, c.is_big, g, t.tyc, t.ntiles, t.nkeys, t.okey, t.bcount, t.ktotal,
+                     (int64_t)c.nx * c.ny, c.node_q, c.filler_q);
   hipLaunchKernelGGL(tile_keyscan_kernel, dim3(1), dim3(kSortThreads), 0, c.stream, t.nkeys, t.ktotal, t.kstart,
                      t.chunks, t.nchunks);
   hipLaunchKernelGGL(tile_place_kernel, dim3(t.nblocks), dim3(kSortThreads), lds, c.stream, c.O, t.per, t.nkeys,

@@ -529,7 +529,6 @@ void evaluate(Ctx &c, bool lambda_update, bool lipschitz, const float2 *ref_now,
     c.prof_used += kEvPerEval;
   }
   if (ev) (void)hipEventRecord(ev[0], c.stream);
-  launch_zero_maps(c);
   launch_scatter(c);
   if (ev) (void)hipEventRecord(ev[1], c.stream);
   launch_finalize_density(c);
@@ -548,7 +547,6 @@ void evaluate(Ctx &c, bool lambda_update, bool lipschitz, const float2 *ref_now,
   if (ev) (void)hipEventRecord(ev[4], c.stream);
   (void)hipStreamWaitEvent(c.stream, c.ev_join, 0);
   launch_combine(c, false, lipschitz, ref_now, ref_prev, grad_prev);
-  launch_count_eval(c);
   if (ev) (void)hipEventRecord(ev[5], c.stream);
 }
 

@@ -241,7 +241,6 @@ struct Ctx {
 };
 
 // ---- launchers (density.hip) ----
-void launch_zero_maps(Ctx &c);
 void launch_scatter(Ctx &c);
 void launch_finalize_density(Ctx &c);
 void launch_gather_combine(Ctx &c, bool recompute_gather, bool lipschitz, const float2 *ref_now,
@@ -274,7 +273,6 @@ void launch_done_check(Ctx &c, int next_iter);
 void launch_done_check_ex(Ctx &c, int next_iter, double *log, int log_cap, int set_iter);
 void launch_nesterov_move_ex(Ctx &c, const float2 *ref, const float2 *grad, const float2 *main_in,
                              float2 *main_out, float2 *ref_unclamped_out, float2 *save_ref, float2 *save_grad);
-void launch_count_eval(Ctx &c);
 int poisson_ld(const Ctx &c);
 // ---- controller.hip ----
 struct CtrlDev;

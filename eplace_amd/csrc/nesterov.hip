@@ -155,9 +155,6 @@ __global__ void done_check_kernel(DevScalars *sc, int next_iter, double *log, in
   if (stop) sc->done = 1;
 }
 
-__global__ void count_eval_kernel(DevScalars *sc) {
-  if (!sc->done) sc->evals += 1;
-}
 
 static int blocks_for(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
 
@@ -205,8 +202,5 @@ void launch_done_check_ex(Ctx &c, int next_iter, double *log, int log_cap, int s
 
 void launch_done_check(Ctx &c, int next_iter) { launch_done_check_ex(c, next_iter, nullptr, 0, 1); }
 
-void launch_count_eval(Ctx &c) {
-  hipLaunchKernelGGL(count_eval_kernel, dim3(1), dim3(1), 0, c.stream, c.sc);
-}
 
 }  // namespace ep

@@ -96,7 +96,7 @@ struct PGemm {
 };
 
 constexpr int PBM = 64, PBN = 64, PK = 32;
-constexpr int kGroups = 4;  // k-groups per block: each runs a quarter of the k steps, then they add up
+constexpr int kGroups = 2;  // k-groups per block: each runs half the k steps, then they add up
 constexpr int kGroupThreads = 256;
 
 // Thread index inside the thread's k-group.

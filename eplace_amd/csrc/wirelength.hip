@@ -322,8 +322,9 @@ static int g_small_blocks(const Ctx &c) {
   const int wpb = kBlock / 64;
   const int groups = c.wl_groups;
   int b = (groups + wpb - 1) / wpb;
-  // 1024 blocks leave room for the transforms running beside this kernel
-  b = b < 1 ? 1 : (b > 1024 ? 1024 : b);
+  // two blocks per CU: the transform GEMM blocks running beside this kernel
+  // on the other stream keep room on every CU
+  b = b < 1 ? 1 : (b > 512 ? 512 : b);
   return (b + kXcds - 1) / kXcds * kXcds;
 }
 static int g_large_blocks(const Ctx &c) {
