@@ -60,10 +60,18 @@ template <int MODEL>
 __device__ __forceinline__ float pin_grad_axis(float x, float xmax, float xmin, float ig, float ep, float em,
                                                float dp, float np, float dm, float nm) {
   if (MODEL == 1) return (float)((double)ep / dp - (double)em / dm);  // LSE
+  // one reciprocal per sum (the division's rounding differs from a/b by an
+  // ulp of double, far below the float result)
   double gp = 0.0, gm = 0.0;
   const double g = ig;
-  if (ep != 0.f) gp = (double)ep / dp * (1.0 + g * ((double)(x - xmax) - (double)np / dp));
-  if (em != 0.f) gm = (double)em / dm * (1.0 - g * ((double)(x - xmin) - (double)nm / dm));
+  if (ep != 0.f) {
+    const double r = 1.0 / (double)dp;
+    gp = (double)ep * r * (1.0 + g * ((double)(x - xmax) - (double)np * r));
+  }
+  if (em != 0.f) {
+    const double r = 1.0 / (double)dm;
+    gm = (double)em * r * (1.0 - g * ((double)(x - xmin) - (double)nm * r));
+  }
   return (float)(gp - gm);
 }
 
@@ -217,19 +225,12 @@ struct LanePins {
   const float2 *pos;
 };
 
+// One 64-lane group from its loaded pin positions (act: the lane holds a
+// pin, idx: the lane's slot).
 template <int D, int MODEL, int EXPM, int GRAD>
-__device__ __forceinline__ void wl_group_d(int g, const LanePins &lp, float igx, float igy,
+__device__ __forceinline__ void wl_group_d(int idx, bool act, float px, float py, float igx, float igy,
                                            float2 *__restrict__ pin_grad, double &hp, double &wv) {
   const int lane = threadIdx.x & 63;
-  const int idx = g * 64 + lane;
-  const int obj = lp.obj[idx];
-  const bool act = obj >= 0;
-  float px = 0.f, py = 0.f;
-  if (act) {
-    const float2 c = lp.pos[obj], o = lp.off[idx];
-    px = c.x + o.x;
-    py = c.y + o.y;
-  }
   auto fmax_ = [](float a, float b) { return fmaxf(a, b); };
   auto fmin_ = [](float a, float b) { return fminf(a, b); };
   auto fadd_ = [](float a, float b) { return a + b; };
@@ -275,15 +276,44 @@ __global__ void __launch_bounds__(kBlock) wl_group_kernel(WlClasses cls, LanePin
   const int per = (total + waves - 1) / waves;
   const int g0 = wid * per, g1 = min(total, g0 + per);
   double hp = 0, wv = 0;
+  // Software pipeline over the wave's groups: the pin slots of group g + 2
+  // and the centre gather of group g + 1 are in flight while group g is
+  // reduced, so the dependent load chain (slot -> object -> centre) is paid
+  // once per wave instead of once per group.
+  const int lane = threadIdx.x & 63;
+  int obj1 = -1, obj2 = -1;
+  float2 off1 = make_float2(0.f, 0.f), off2 = off1, c1 = off1;
+  if (g0 < g1) {
+    obj1 = lp.obj[g0 * 64 + lane];
+    off1 = lp.off[g0 * 64 + lane];
+  }
+  if (g0 + 1 < g1) {
+    obj2 = lp.obj[(g0 + 1) * 64 + lane];
+    off2 = lp.off[(g0 + 1) * 64 + lane];
+  }
+  if (obj1 >= 0) c1 = lp.pos[obj1];
   for (int g = g0; g < g1; ++g) {
+    const int obj = obj1;
+    const float2 off = off1, c0 = c1;
+    obj1 = obj2;
+    off1 = off2;
+    obj2 = -1;
+    if (g + 2 < g1) {
+      obj2 = lp.obj[(g + 2) * 64 + lane];
+      off2 = lp.off[(g + 2) * 64 + lane];
+    }
+    if (obj1 >= 0) c1 = lp.pos[obj1];
+    const bool act = obj >= 0;
+    const float px = act ? c0.x + off.x : 0.f, py = act ? c0.y + off.y : 0.f;
+    const int idx = g * 64 + lane;
     int c = 0;
     while (c < 4 && g >= cls.group_start[c + 1]) ++c;
     switch (c) {
-      case 0: wl_group_d<2, MODEL, EXPM, GRAD>(g, lp, igx, igy, pin_grad, hp, wv); break;
-      case 1: wl_group_d<4, MODEL, EXPM, GRAD>(g, lp, igx, igy, pin_grad, hp, wv); break;
-      case 2: wl_group_d<8, MODEL, EXPM, GRAD>(g, lp, igx, igy, pin_grad, hp, wv); break;
-      case 3: wl_group_d<16, MODEL, EXPM, GRAD>(g, lp, igx, igy, pin_grad, hp, wv); break;
-      default: wl_group_d<32, MODEL, EXPM, GRAD>(g, lp, igx, igy, pin_grad, hp, wv); break;
+      case 0: wl_group_d<2, MODEL, EXPM, GRAD>(idx, act, px, py, igx, igy, pin_grad, hp, wv); break;
+      case 1: wl_group_d<4, MODEL, EXPM, GRAD>(idx, act, px, py, igx, igy, pin_grad, hp, wv); break;
+      case 2: wl_group_d<8, MODEL, EXPM, GRAD>(idx, act, px, py, igx, igy, pin_grad, hp, wv); break;
+      case 3: wl_group_d<16, MODEL, EXPM, GRAD>(idx, act, px, py, igx, igy, pin_grad, hp, wv); break;
+      default: wl_group_d<32, MODEL, EXPM, GRAD>(idx, act, px, py, igx, igy, pin_grad, hp, wv); break;
     }
   }
   hp = block_sum(hp);
@@ -322,9 +352,9 @@ static int g_small_blocks(const Ctx &c) {
   const int wpb = kBlock / 64;
   const int groups = c.wl_groups;
   int b = (groups + wpb - 1) / wpb;
-  // two blocks per CU: the transform GEMM blocks running beside this kernel
-  // on the other stream keep room on every CU
-  b = b < 1 ? 1 : (b > 512 ? 512 : b);
+  // four blocks per CU leave room for the transform GEMM blocks running
+  // beside this kernel on the other stream
+  b = b < 1 ? 1 : (b > 1024 ? 1024 : b);
   return (b + kXcds - 1) / kXcds * kXcds;
 }
 static int g_large_blocks(const Ctx &c) {
