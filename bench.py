@@ -60,37 +60,76 @@ def poisson_flops(info):
             + 2.0 * hx * Y * (2 * hx) * 2)
 
 
-def roofline(phase, info, design, poisson_mode=0):
-    """Algorithmic work per launch-group of the dominant phase."""
-    O, N, P, E = info["num_objects"], info["num_nodes"], info["num_pins"], info["num_nets"]
+# Algorithmic HBM bytes of one evaluation's phase (the bytes the operation
+# must move once, independent of how it is implemented).
+PHASE_BYTES = {
+    # per object: x, y, w, h, kind, big flag (18 B); per bin: clear and accumulate two int64 maps
+    "scatter": lambda O, N, P, nb: 18.0 * O + 32.0 * nb,
+    # per bin: two int64 maps, terminal and base maps in, rho out
+    "finalize": lambda O, N, P, nb: 28.0 * nb,
+    # per pin: object index (4), offset (8), gathered centre (8), gradient out (8)
+    "wirelength": lambda O, N, P, nb: 28.0 * P,
+    # per pin: gradient slot index (4) and gathered gradient (8); per node: pin range (4), sum out (8)
+    "node_sums": lambda O, N, P, nb: 12.0 * P + 12.0 * N,
+    # per object: sorted index (4), footprint (16), gradient out (8); per bin: Ex, Ey once
+    "field_gather": lambda O, N, P, nb: 28.0 * O + 8.0 * nb,
+    # per object: kind, stage flag, pin count, area, density and wirelength gradients, previous
+    # point and gradient, current point in (50 B), total gradient out (8 B)
+    "combine": lambda O, N, P, nb: 58.0 * O,
+}
+
+PHASE_KERNELS = {
+    "scatter": "tile_hist_kernel + tile_keyscan_kernel + tile_place_kernel + tile_scatter_kernel + scatter_big_kernel",
+    "finalize": "finalize_density_kernel + finish_density_kernel",
+    "wirelength": "wl_group_kernel<0,0,1> + wl_large_kernel<0,0,1>",
+    "node_sums": "reduce_wl_kernel + node_wl_kernel",
+    "field_gather": "gather_tile_kernel + gather_big_kernel",
+    "combine": "gather_combine_kernel (after the second stream joins)",
+}
+
+
+def phase_traffic(kernel_key, info):
+    """HBM bytes of one evaluation's phase (all its launches) from the
+    committed rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE scaled by a
+    plain copy of known size), when it was measured on this workload;
+    otherwise None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_per_launch.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    w = t.get("workload", {})
+    if w.get("num_objects") != info["num_objects"] or w.get("bins") != [info["bins_x"], info["bins_y"]]:
+        return None
+    v = t.get("phases", {}).get(kernel_key)
+    return None if v is None else float(v["bytes"])
+
+
+def per_launch(v, launches):
+    return None if v is None else v / launches
+
+
+def roofline(name, seconds, info, poisson_mode=0):
+    """Achieved rate of one evaluation phase against its roofline: HBM bytes
+    for the streaming phases, MFMA flops for the Poisson transforms."""
+    O, N, P = info["num_objects"], info["num_nodes"], info["num_pins"]
     nb = info["bins_x"] * info["bins_y"]
-    sec = phase["seconds"]
-    name = phase["name"]
     if name == "poisson":
         flops = poisson_flops(info)
         if poisson_mode == 1:  # three bf16 products per FP32 product
             flops, peak, kern = 3 * flops, BF16_MFMA_PEAK_TFLOPS, "pgemm_kernel<1> x4 (split-BF16)"
         else:
             peak, kern = FP32_MFMA_PEAK_TFLOPS, "pgemm_kernel<0> x4 (FP32)"
-        ach = flops / sec / 1e12
-        return {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
-                "frac": ach / peak, "traffic": None, "kernel": kern, "work_per_launch": flops / 4}
-    if name == "wirelength":
-        # per pin: obj idx, dx, dy, gathered centre (20 B) + pin gradient (8 B); per net: start + index
-        byts = 28.0 * P + 8.0 * E
-    elif name == "scatter":
-        # per object: x, y, w, h, kind, big flag; per bin: clear and accumulate two int64 maps
-        byts = 18.0 * O + 32.0 * nb
-    elif name == "finalize":
-        # per bin: two int64 maps, terminal and base maps in, rho out
-        byts = 28.0 * nb
-    else:
-        # per object: geometry, kind, pins count, area (26 B), previous y and g (16 B), outputs (24 B);
-        # per pin: CSR index + pin gradient (12 B); the two field maps once
-        byts = 66.0 * O + 12.0 * P + 8.0 * nb
-    ach = byts / sec / 1e9
+        ach = flops / seconds / 1e12
+        return {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
+                "traffic": per_launch(phase_traffic(name, info), 4), "kernel": kern, "launches": 4,
+                "work_per_launch": flops / 4, "seconds_per_launch": seconds / 4}
+    byts = PHASE_BYTES[name](O, N, P, nb)
+    ach = byts / seconds / 1e9
     return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-            "traffic": None, "kernel": name, "work_per_launch": byts}
+            "traffic": phase_traffic(name, info), "kernel": PHASE_KERNELS[name], "launches": 1,
+            "work_per_launch": byts, "seconds_per_launch": seconds}
 
 
 def cpu_baseline(design, method, iters, init_positions):
@@ -158,7 +197,7 @@ def main():
     ap.add_argument("--no-converge", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-controller convergence run")
-    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--cpu-iters", type=int, default=10)
     ap.add_argument("--no-qp", action="store_true", help="skip the quadratic-placement run")
     ap.add_argument("--no-qp-cpu", action="store_true", help="skip the reference QP CPU baseline")
     ap.add_argument("--poisson-mode", type=int, default=1, choices=[0, 1],
@@ -212,7 +251,7 @@ def main():
     restart()
     device_sync()
     t0 = time.perf_counter()
-    opt = EplaceNesterovOpt(gp, method=args.method, max_iterations=args.steps - 1)
+    opt = EplaceNesterovOpt(gp, method=args.method, max_iterations=args.steps - 1, log=False)
     r = opt.opt(profile=True)
     device_sync()
     dt = time.perf_counter() - t0
@@ -220,8 +259,12 @@ def main():
     print(f"[bench] timed {steps} iterations in {dt:.3f}s, {r['evaluations']} evaluations, "
           f"{r['backtracks']} backtracks", file=sys.stderr, flush=True)
     ph = r["phase_seconds"]
-    dominant = max(ph, key=ph.get)
-    roof = roofline({"name": dominant, "seconds": ph[dominant]}, info, d, args.poisson_mode)
+    # dominant phase: the largest measured time per evaluation among the
+    # kernel phases (the combine also holds the wait for the second stream)
+    kernel_phases = [k for k in ph if k != "combine"]
+    dominant = max(kernel_phases, key=ph.get)
+    roof = roofline(dominant, ph[dominant], info, args.poisson_mode)
+    roof_all = {k: roofline(k, ph[k], info, args.poisson_mode) for k in ph}
 
     conv = None
     if not args.no_converge:
@@ -230,7 +273,7 @@ def main():
         device_sync()
         t0 = time.perf_counter()
         gp.initialization()
-        rc = EplaceNesterovOpt(gp, method=args.method, target_overflow=0.10).opt()
+        rc = EplaceNesterovOpt(gp, method=args.method, target_overflow=0.10, log=False).opt()
         device_sync()
         tc = time.perf_counter() - t0
         conv = {"seconds_to_overflow_0.10": tc, "iterations": rc["iterations"], "evaluations": rc["evaluations"],
@@ -319,6 +362,8 @@ def main():
                    "method": f"plain {args.method} Nesterov (device-resident)", "cells": cells,
                    "bins": [info["bins_x"], info["bins_y"]]},
         "roofline": roof,
+        "roofline_phases": {k: {"bound": v["bound"], "achieved": v["achieved"], "unit": v["unit"],
+                                "frac": v["frac"], "traffic": v["traffic"]} for k, v in roof_all.items()},
         "phase_ms_per_evaluation": {k: v * 1e3 for k, v in ph.items()},
         "converge": conv,
         "qp": qp,

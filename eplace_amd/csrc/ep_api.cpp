@@ -537,6 +537,7 @@ void evaluate(Ctx &c, bool lambda_update, bool lipschitz, const float2 *ref_now,
   (void)hipStreamWaitEvent(c.stream2, c.ev_fork, 0);
   if (ev) (void)hipEventRecord(ev[6], c.stream2);
   launch_wirelength(c, true, c.stream2);
+  if (ev) (void)hipEventRecord(ev[8], c.stream2);
   launch_reduce_wirelength(c, lambda_update ? 1 : 0, c.stream2);
   launch_node_wl(c, c.stream2);
   if (ev) (void)hipEventRecord(ev[7], c.stream2);
@@ -550,11 +551,12 @@ void evaluate(Ctx &c, bool lambda_update, bool lipschitz, const float2 *ref_now,
   if (ev) (void)hipEventRecord(ev[5], c.stream);
 }
 
-// Phase times of one evaluation from its events: scatter, finalise, wiring,
-// Poisson, field gather + combine (the last includes any wait for the wiring).
+// Phase times of one evaluation from its events: scatter, finalise, wiring
+// kernels, node sums, Poisson, field gather, combine (the combine includes
+// any wait for the second stream).
 void phase_times(const hipEvent_t *ev, double *out) {
-  static const int pairs[5][2] = {{0, 1}, {1, 2}, {6, 7}, {2, 3}, {3, 5}};
-  for (int k = 0; k < 5; ++k) {
+  static const int pairs[kPhases][2] = {{0, 1}, {1, 2}, {6, 8}, {8, 7}, {2, 3}, {3, 4}, {4, 5}};
+  for (int k = 0; k < kPhases; ++k) {
     float ms = 0;
     (void)hipEventElapsedTime(&ms, ev[pairs[k][0]], ev[pairs[k][1]]);
     out[k] = ms * 1e-3;
@@ -779,15 +781,15 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
   auto t1 = std::chrono::steady_clock::now();
   if (c.profiling) {
     c.profiling = false;
-    double acc[5] = {0, 0, 0, 0, 0};
+    double acc[kPhases] = {};
     size_t n = c.prof_used / kEvPerEval;
     for (size_t e = 0; e < n; ++e) {
-      double t[5];
+      double t[kPhases];
       ep::phase_times(&c.prof_events[kEvPerEval * e], t);
-      for (int k = 0; k < 5; ++k) acc[k] += t[k];
+      for (int k = 0; k < kPhases; ++k) acc[k] += t[k];
     }
-    for (int k = 0; k < 5; ++k) o.phase_seconds[k] = acc[k];
-    o.phase_seconds[5] = (double)n;
+    for (int k = 0; k < kPhases; ++k) o.phase_seconds[k] = acc[k];
+    o.phase_seconds[kPhases] = (double)n;
   }
   if (cap > 0) {
     EP_HIP(hipMemcpy(o.log, dlog, (size_t)cap * EP_LOG_COLS * sizeof(double), hipMemcpyDeviceToHost));
@@ -1091,7 +1093,7 @@ int ep_write(ep_ctx *x, int32_t which, const void *host, int64_t count) {
 
 int ep_last_timings(ep_ctx *x, double *s) {
   if (!x || !s) return EP_ERR_ARG;
-  for (int k = 0; k < 5; ++k) s[k] = x->c.last_timings[k];
+  for (int k = 0; k < ep::kPhases; ++k) s[k] = x->c.last_timings[k];
   return EP_OK;
 }
 

@@ -48,8 +48,11 @@ constexpr int kBlock = 256;
 constexpr int kFinish = 1024;  // single-block kernels that sum per-block partials
 constexpr int kNetLaneMax = 32;  // nets above this get a wave each; the rest a power-of-two lane group each
 constexpr int kBigObjectBins = 16;      // objects that may cover more bins get a block each
-constexpr int kXcds = 8;
-constexpr int kEvPerEval = 8;  // phase events recorded per evaluation                // MI355X: workgroups go to the 8 XCDs round-robin
+constexpr int kXcds = 8;  // MI355X: workgroups go to the 8 XCDs round-robin
+constexpr int kEvPerEval = 9;  // phase events recorded per evaluation
+// timed phases of an evaluation: scatter, density finalise, wiring kernels,
+// node sums of the pin gradients, Poisson solve, field gather, combine
+constexpr int kPhases = 7;
 
 // Renumbers blocks so that the blocks of each XCD cover one contiguous
 // 1/8 of the grid (block b runs on XCD b % 8).  Work ranges that are
@@ -145,10 +148,10 @@ struct Ctx {
   DevScalars *h_sc = nullptr;
   // per-evaluation phase events: 0 start, 1 scatter end, 2 finalise end,
   // 3 Poisson end, 4 field gather end, 5 combine end (stream), 6 wiring
-  // start, 7 wiring end (stream2)
+  // start, 8 wiring kernels end, 7 node sums end (stream2)
   hipEvent_t ev[kEvPerEval] = {};
-  double last_timings[5] = {};
-  // event pool for per-phase timing inside the optimiser loop (6 per evaluation)
+  double last_timings[kPhases] = {};
+  // event pool for per-phase timing inside the optimiser loop (kEvPerEval per evaluation)
   std::vector<hipEvent_t> prof_events;
   size_t prof_used = 0;
   bool profiling = false;

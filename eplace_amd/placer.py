@@ -44,6 +44,12 @@ def design_struct(d: Design):
     return sd, arrays
 
 
+# timed phases of a gradient evaluation, in the order the library reports them
+# (scatter and density finalise; on the second stream the wiring kernels and
+# the node sums of their pin gradients; then the Poisson solve, the field
+# gather and the combine, which includes any wait for the second stream)
+PHASES = ("scatter", "finalize", "wirelength", "node_sums", "poisson", "field_gather", "combine")
+
 class EPlacer2D:
     """GPU global-placement evaluator (EPlacer_2D)."""
 
@@ -210,9 +216,9 @@ class EPlacer2D:
         return phi, ex, ey
 
     def last_timings(self) -> dict:
-        t = np.zeros(5, np.float64)
+        t = np.zeros(len(PHASES), np.float64)
         self._check(self.lib.ep_last_timings(self.h, _ptr(t)))
-        return dict(zip(["scatter", "finalize", "wirelength", "poisson", "gather_combine"], t.tolist()))
+        return dict(zip(PHASES, t.tolist()))
 
 
 class EplaceNesterovOpt:
@@ -245,7 +251,7 @@ class EplaceNesterovOpt:
     def opt(self, profile: bool = False) -> dict:
         cap = self.max_iterations + 2 if self.keep_log else 0
         log = np.zeros((max(cap, 1), L.EP_LOG_COLS), np.float64)
-        phases = np.zeros(6, np.float64)
+        phases = np.zeros(len(PHASES) + 1, np.float64)
         p = L.EPOptParams(method=self.METHODS[self.method], target_overflow=self.target_overflow,
                           max_iterations=self.max_iterations, max_backtracks=self.max_backtracks,
                           sync_every=self.sync_every, log_capacity=cap, log=_ptr(log) if cap else None,
@@ -255,10 +261,9 @@ class EplaceNesterovOpt:
         self.placer._check(self.placer.lib.ep_nesterov_run(self.placer.h, C.byref(p), C.byref(r)))
         res = {k: getattr(r, k) for k, _ in L.EPRunStats._fields_ if k != "pad"}
         if profile:
-            n = max(phases[5], 1.0)
-            res["phase_seconds"] = dict(zip(["scatter", "finalize", "wirelength", "poisson", "gather_combine"],
-                                            (phases[:5] / n).tolist()))
-            res["phase_evaluations"] = int(phases[5])
+            n = max(phases[-1], 1.0)
+            res["phase_seconds"] = dict(zip(PHASES, (phases[:-1] / n).tolist()))
+            res["phase_evaluations"] = int(phases[-1])
         self.log = log[: res["iterations"]] if cap else None
         if self.placer.placementStage == MGP:
             self.placer.mGPIterationCount = res["iterations"]

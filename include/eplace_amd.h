@@ -118,9 +118,10 @@ typedef struct ep_opt_params {
   int32_t sync_every;       /* host polls the stop flag every this many iterations */
   int32_t log_capacity;     /* rows available in log (may be 0) */
   double *log;              /* optional [log_capacity][EP_LOG_COLS], columns EP_LOG_* */
-  double *phase_seconds;    /* optional [6]: summed kernel time of the full evaluations per phase
-                               (scatter, density finalise, wirelength, poisson, gather+combine) and
-                               the number of evaluations timed; HIP events on the engine's stream */
+  double *phase_seconds;    /* optional [8]: summed time of the full evaluations per phase
+                               (scatter, density finalise, wiring kernels, node sums of the pin
+                               gradients, poisson, field gather, combine) and the number of
+                               evaluations timed; HIP events on the streams the phases run on */
   const ep_ctrl *controller; /* method 2: the step / lambda controller (same device) */
 } ep_opt_params;
 
@@ -196,8 +197,9 @@ int ep_read(ep_ctx *ctx, int32_t which, void *host, int64_t count);
 int ep_write(ep_ctx *ctx, int32_t which, const void *host, int64_t count);
 
 /* Wall time in seconds of the last ep_total_gradient_update's kernels, split
- * per phase: [scatter, density finalise, wirelength, poisson, gather+combine]. */
-int ep_last_timings(ep_ctx *ctx, double *seconds5);
+ * per phase: [scatter, density finalise, wiring kernels, node sums of the pin
+ * gradients, poisson, field gather, combine]. */
+int ep_last_timings(ep_ctx *ctx, double *seconds7);
 
 /* ---------------------------------------------------------------------------
  * Standard-cell legalisation (host, no device work).  These take the design
