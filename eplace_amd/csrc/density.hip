@@ -177,6 +177,9 @@ __global__ void __launch_bounds__(kSortThreads) tile_place_kernel(
   }
 }
 
+// Chunks of one key (kind, tile) accumulate in an LDS window of the tile plus
+// its high-side halo; footprint bins outside the window go to global memory
+// directly.
 __global__ void __launch_bounds__(kBlock) tile_scatter_kernel(const int4 *__restrict__ chunks,
                                                                const int *__restrict__ nchunks,
                                                                const float4 *__restrict__ vals, Grid g, int tyc,
@@ -358,7 +361,7 @@ __global__ void __launch_bounds__(kBlock) gather_tile_kernel(const int4 *__restr
           const float a = (float)((double)(f.sx * f.sy) * ov);
           const int li = i - ib, lj = j - jb;
           float e0, e1;
-          if (li < kTileWin && lj < kTileWin) {
+          if (li >= 0 && li < kTileWin && lj >= 0 && lj < kTileWin) {
             e0 = wex[li * kTileWin + lj];
             e1 = wey[li * kTileWin + lj];
           } else {
@@ -511,10 +514,11 @@ void launch_scatter(Ctx &c) {
   Grid g = c.grid();
   double scale = ldexp(1.0, c.shift);
   TileSort &t = c.tiles;
+  const int64_t nb = (int64_t)c.nx * c.ny;
   const size_t lds = (size_t)t.nkeys * sizeof(uint32_t);
   hipLaunchKernelGGL(tile_hist_kernel, dim3(t.nblocks), dim3(kSortThreads), lds, c.stream, c.O, t.per, c.pos, c.w,
-                     c.h, c.kind, c.is_big, g, t.tyc, t.ntiles, t.nkeys, t.okey, t.bcount, t.ktotal,
-                     (int64_t)c.nx * c.ny, c.node_q, c.filler_q);
+                     c.h, c.kind, c.is_big, g, t.tyc, t.ntiles, t.nkeys, t.okey, t.bcount, t.ktotal, nb, c.node_q,
+                     c.filler_q);
   hipLaunchKernelGGL(tile_keyscan_kernel, dim3(1), dim3(kSortThreads), 0, c.stream, t.nkeys, t.ktotal, t.kstart,
                      t.chunks, t.nchunks);
   hipLaunchKernelGGL(tile_place_kernel, dim3(t.nblocks), dim3(kSortThreads), lds, c.stream, c.O, t.per, t.nkeys,
