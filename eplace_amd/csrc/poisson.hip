@@ -252,9 +252,11 @@ __device__ __forceinline__ void mma(const Smem16 &s, int buf, floatx4 (&acc)[2][
 
 // acc += A[rows, k_begin..k_end) B[cols, k_begin..k_end)^T, the k steps
 // split between the block's k-groups (group g takes steps g, g + kGroups,
-// ...).  Each group has its own LDS double buffer with the next step's
-// global loads in flight during the multiply; every group passes the same
-// number of block barriers (one per round), also when it has no step left.
+// ...).  Each group has its own LDS double buffer and two register stages:
+// while round r multiplies from LDS, the global loads of rounds r + 1 and
+// r + 2 are in flight (round r + 1's go to LDS at the end of round r).
+// Every group passes the same number of block barriers (one per round),
+// also when it has no step left.
 template <class ST, class SM>
 __device__ __forceinline__ void run_k(const PGemm &p, int z, const float *B, float sb, float sub, int row0,
                                       int col0, int k_begin, int k_end, SM &s, floatx4 (&acc)[2][2]) {
@@ -262,19 +264,25 @@ __device__ __forceinline__ void run_k(const PGemm &p, int z, const float *B, flo
   const int steps = (k_end - k_begin + PK - 1) / PK;
   const int rounds = (steps + kGroups - 1) / kGroups;
   auto kstep = [&](int r) { return k_begin + (r * kGroups + grp) * PK; };
-  ST st;
-  if (kstep(0) < k_end) load_stage(p, z, B, sb, sub, row0, col0, kstep(0), st);
+  auto have = [&](int r) { return kstep(r) < k_end; };
+  ST even, odd;  // register stages of the even and the odd rounds
+  if (have(0)) load_stage(p, z, B, sb, sub, row0, col0, kstep(0), even);
+  if (have(1)) load_stage(p, z, B, sb, sub, row0, col0, kstep(1), odd);
   __syncthreads();  // the previous pass may still read buffer 0
-  if (kstep(0) < k_end) store_stage(s, 0, st);
+  if (have(0)) store_stage(s, 0, even);
   __syncthreads();
-  int buf = 0;
-  for (int r = 0; r < rounds; ++r) {
-    const bool have = kstep(r) < k_end, next = kstep(r + 1) < k_end;
-    if (next) load_stage(p, z, B, sb, sub, row0, col0, kstep(r + 1), st);
-    if (have) mma(s, buf, acc);
-    if (next) store_stage(s, buf ^ 1, st);
+  for (int r = 0; r < rounds; r += 2) {
+    // round r: buffer 0 holds step r, `odd` step r + 1, `even` is free
+    if (have(r + 2)) load_stage(p, z, B, sb, sub, row0, col0, kstep(r + 2), even);
+    if (have(r)) mma(s, 0, acc);
+    if (have(r + 1)) store_stage(s, 1, odd);
     __syncthreads();
-    buf ^= 1;
+    if (r + 1 >= rounds) break;
+    // round r + 1: buffer 1 holds step r + 1, `even` step r + 2, `odd` is free
+    if (have(r + 3)) load_stage(p, z, B, sb, sub, row0, col0, kstep(r + 3), odd);
+    if (have(r + 1)) mma(s, 1, acc);
+    if (have(r + 2)) store_stage(s, 0, even);
+    __syncthreads();
   }
 }
 
