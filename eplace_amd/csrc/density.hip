@@ -56,10 +56,15 @@ __device__ __forceinline__ double overlap_len(double x1, double x2, double x3, d
   double s = fmax(x1, x3), e = fmin(x2, x4);
   return s >= e ? 0.0 : s - e;
 }
+// Bin boundaries are recomputed the way the grid built them (i * bsx + llx,
+// then + bsx, float arithmetic without contraction), so they are the stored
+// values without a memory read.
 __device__ __forceinline__ double bin_overlap(const Grid &g, int i, int j, const Foot &f) {
-  double hl = overlap_len(g.bllx[i], g.burx[i], f.l, f.r);
+  const float lx = (float)i * g.bsx + g.core_llx;
+  double hl = overlap_len(lx, lx + g.bsx, f.l, f.r);
   if (hl == 0) return 0;
-  double vl = overlap_len(g.blly[j], g.bury[j], f.b, f.t);
+  const float ly = (float)j * g.bsy + g.core_lly;
+  double vl = overlap_len(ly, ly + g.bsy, f.b, f.t);
   if (vl == 0) return 0;
   return hl * vl;
 }

@@ -464,8 +464,6 @@ int create(Ctx &c, const ep_design *d, const ep_params *p) {
             dev_upload(c, c.node_pin_start, nps.data(), c.N + 1) &&
             dev_upload(c, c.node_pin_lane, npl.data(), npl.size()) &&
             dev_zero(c, c.pin_grad, (size_t)c.grad_lanes + 1) &&
-            dev_upload(c, c.bllx, s.bllx.data(), c.nx) && dev_upload(c, c.burx, s.burx.data(), c.nx) &&
-            dev_upload(c, c.blly, s.blly.data(), c.ny) && dev_upload(c, c.bury, s.bury.data(), c.ny) &&
             dev_zero(c, c.node_q, nb) && dev_zero(c, c.filler_q, nb) &&
             dev_upload(c, c.term_map, s.term_map.data(), nb) && dev_upload(c, c.base_map, s.base_map.data(), nb) &&
             dev_zero(c, c.gd, c.O) && dev_zero(c, c.gwl, c.O) && dev_zero(c, c.gtot, c.O) &&
@@ -496,7 +494,7 @@ void destroy(Ctx &c) {
   tile_sort_destroy(c);
   void *ptrs[] = {c.pos, c.w, c.h, c.area, c.kind, c.in_stage, c.npins, c.big, c.is_big,
                   c.net_start, c.pin_obj, c.pin_dx, c.pin_dy, c.pin_net, c.lane_obj, c.lane_off, c.large_nets, c.node_pin_start,
-                  c.node_pin_lane, c.pin_grad, c.bllx, c.burx, c.blly, c.bury, c.node_q, c.filler_q, c.term_map,
+                  c.node_pin_lane, c.pin_grad, c.node_q, c.filler_q, c.term_map,
                   c.base_map, c.rho, c.phi, c.ex, c.ey, c.gd, c.gwl, c.gtot, c.cur_main, c.cur_ref, c.last_ref,
                   c.last_grad, c.new_main, c.new_ref, c.cur_grad, c.sc, c.partials, c.tickets};
   for (void *p : ptrs)

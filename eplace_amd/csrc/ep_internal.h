@@ -77,7 +77,6 @@ struct Grid {
   int nx, ny;
   float bsx, bsy, core_llx, core_lly;
   float bin_area, inv_bin_area, td;
-  const float *bllx, *burx, *blly, *bury;
 };
 
 // Device-resident scalars of the evaluator and optimiser.
@@ -208,7 +207,6 @@ struct Ctx {
                                       // entry grad_lanes stays zero (pins of one-pin nets)
   int32_t large_lane_off = 0;         // pin_grad entry of large-net pin p = p + large_lane_off
   // bins
-  float *bllx = nullptr, *burx = nullptr, *blly = nullptr, *bury = nullptr;
   unsigned long long *node_q = nullptr, *filler_q = nullptr;
   float *term_map = nullptr, *base_map = nullptr;
   float *rho = nullptr, *phi = nullptr, *ex = nullptr, *ey = nullptr;
@@ -235,10 +233,6 @@ struct Ctx {
     g.bin_area = bsx * bsy;
     g.inv_bin_area = (float)(1.0 / (double)(bsx * bsy));
     g.td = td;
-    g.bllx = bllx;
-    g.burx = burx;
-    g.blly = blly;
-    g.bury = bury;
     return g;
   }
 };
