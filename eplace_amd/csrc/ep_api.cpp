@@ -502,6 +502,8 @@ void destroy(Ctx &c) {
   for (auto &e : c.ev)
     if (e) (void)hipEventDestroy(e);
   for (auto &e : c.prof_events) (void)hipEventDestroy(e);
+  for (auto &g : c.eval_graphs) (void)hipGraphExecDestroy(g.exec);
+  c.eval_graphs.clear();
   if (c.h_sc) (void)hipHostFree(c.h_sc);
   if (c.ev_fork) (void)hipEventDestroy(c.ev_fork);
   if (c.ev_join) (void)hipEventDestroy(c.ev_join);
@@ -513,10 +515,70 @@ void destroy(Ctx &c) {
   delete c.rng;
 }
 
-// One full evaluation of totalGradientUpdate at the current positions.  The
-// wiring kernels and the node sums of their pin gradients need only gamma
-// (set by the density finalise), so they run on the second stream beside
-// the Poisson solve and the field gather; the combine waits for them.
+// The kernels of one evaluation of totalGradientUpdate.  The wiring kernels
+// and the node sums of their pin gradients need only gamma (set by the
+// density finalise), so they run on the second stream beside the Poisson
+// solve and the field gather; the combine waits for them.
+static void enqueue_evaluation(Ctx &c, bool lambda_update, bool lipschitz, const float2 *ref_now,
+                               const float2 *ref_prev, const float2 *grad_prev, hipEvent_t *ev) {
+  if (ev) (void)hipEventRecord(ev[0], c.stream);
+  launch_scatter(c);
+  if (ev) (void)hipEventRecord(ev[1], c.stream);
+  launch_finalize_density(c);
+  if (ev) (void)hipEventRecord(ev[2], c.stream);
+  (void)hipEventRecord(c.ev_fork, c.stream);
+  // The transform GEMMs go first: their blocks (8 waves, 64 KB LDS) take one
+  // CU each before the wiring kernel's blocks fill the rest; launched after
+  // it, they waited for wiring blocks to retire.
+  launch_poisson(c, false);
+  if (ev) (void)hipEventRecord(ev[3], c.stream);
+  (void)hipStreamWaitEvent(c.stream2, c.ev_fork, 0);
+  if (ev) (void)hipEventRecord(ev[6], c.stream2);
+  launch_wirelength(c, true, c.stream2);
+  if (ev) (void)hipEventRecord(ev[8], c.stream2);
+  launch_reduce_wirelength(c, lambda_update ? 1 : 0, c.stream2);
+  launch_node_wl(c, c.stream2);
+  if (ev) (void)hipEventRecord(ev[7], c.stream2);
+  (void)hipEventRecord(c.ev_join, c.stream2);
+  launch_field_gather(c);
+  if (ev) (void)hipEventRecord(ev[4], c.stream);
+  (void)hipStreamWaitEvent(c.stream, c.ev_join, 0);
+  launch_combine(c, false, lipschitz, ref_now, ref_prev, grad_prev);
+  if (ev) (void)hipEventRecord(ev[5], c.stream);
+}
+
+// The evaluation's graph for these arguments, captured on first use (the
+// second stream joins the capture through the fork event and leaves it at
+// the join).  nullptr when capture is not available.
+static hipGraphExec_t eval_graph(Ctx &c, bool lambda_update, bool lipschitz, const float2 *ref_now,
+                                 const float2 *ref_prev, const float2 *grad_prev) {
+  for (const Ctx::EvalGraph &g : c.eval_graphs)
+    if (g.lambda_update == lambda_update && g.lipschitz == lipschitz && g.ref_now == ref_now &&
+        g.ref_prev == ref_prev && g.grad_prev == grad_prev)
+      return g.exec;
+  if (!c.graphs_ok) return nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    c.graphs_ok = false;
+    return nullptr;
+  }
+  enqueue_evaluation(c, lambda_update, lipschitz, ref_now, ref_prev, grad_prev, nullptr);
+  hipError_t e = hipStreamEndCapture(c.stream, &graph);
+  if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    c.graphs_ok = false;
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  c.eval_graphs.push_back({lambda_update, lipschitz, ref_now, ref_prev, grad_prev, exec});
+  return exec;
+}
+
+// One full evaluation of totalGradientUpdate at the current positions:
+// replayed from its graph, or launched kernel by kernel when its phases are
+// timed.
 void evaluate(Ctx &c, bool lambda_update, bool lipschitz, const float2 *ref_now, const float2 *ref_prev,
               const float2 *grad_prev, bool timed) {
   hipEvent_t *ev = nullptr;
@@ -526,27 +588,11 @@ void evaluate(Ctx &c, bool lambda_update, bool lipschitz, const float2 *ref_now,
     ev = &c.prof_events[c.prof_used];
     c.prof_used += kEvPerEval;
   }
-  if (ev) (void)hipEventRecord(ev[0], c.stream);
-  launch_scatter(c);
-  if (ev) (void)hipEventRecord(ev[1], c.stream);
-  launch_finalize_density(c);
-  if (ev) (void)hipEventRecord(ev[2], c.stream);
-  (void)hipEventRecord(c.ev_fork, c.stream);
-  (void)hipStreamWaitEvent(c.stream2, c.ev_fork, 0);
-  if (ev) (void)hipEventRecord(ev[6], c.stream2);
-  launch_wirelength(c, true, c.stream2);
-  if (ev) (void)hipEventRecord(ev[8], c.stream2);
-  launch_reduce_wirelength(c, lambda_update ? 1 : 0, c.stream2);
-  launch_node_wl(c, c.stream2);
-  if (ev) (void)hipEventRecord(ev[7], c.stream2);
-  (void)hipEventRecord(c.ev_join, c.stream2);
-  launch_poisson(c, false);
-  if (ev) (void)hipEventRecord(ev[3], c.stream);
-  launch_field_gather(c);
-  if (ev) (void)hipEventRecord(ev[4], c.stream);
-  (void)hipStreamWaitEvent(c.stream, c.ev_join, 0);
-  launch_combine(c, false, lipschitz, ref_now, ref_prev, grad_prev);
-  if (ev) (void)hipEventRecord(ev[5], c.stream);
+  if (!ev) {
+    hipGraphExec_t g = eval_graph(c, lambda_update, lipschitz, ref_now, ref_prev, grad_prev);
+    if (g && hipGraphLaunch(g, c.stream) == hipSuccess) return;
+  }
+  enqueue_evaluation(c, lambda_update, lipschitz, ref_now, ref_prev, grad_prev, ev);
 }
 
 // Phase times of one evaluation from its events: scatter, finalise, wiring

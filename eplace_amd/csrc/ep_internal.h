@@ -152,6 +152,16 @@ struct Ctx {
   double last_timings[kPhases] = {};
   // event pool for per-phase timing inside the optimiser loop (kEvPerEval per evaluation)
   std::vector<hipEvent_t> prof_events;
+  // Captured evaluations (HIP graphs), one per distinct set of launch
+  // arguments; an evaluation without phase events replays its graph with
+  // one launch instead of ~20 kernel launches and event operations.
+  struct EvalGraph {
+    bool lambda_update, lipschitz;
+    const float2 *ref_now, *ref_prev, *grad_prev;
+    hipGraphExec_t exec;
+  };
+  std::vector<EvalGraph> eval_graphs;
+  bool graphs_ok = true;  // cleared if capture or instantiation fails
   size_t prof_used = 0;
   bool profiling = false;
 
