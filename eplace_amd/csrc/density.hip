@@ -421,7 +421,8 @@ __global__ void __launch_bounds__(kBlock) gather_combine_kernel(
     const int32_t *__restrict__ node_pin_lane, const float2 *__restrict__ pin_grad, const float2 *gd, float2 *gwl,
     float2 *gtot, int recompute, int lipschitz, const uint8_t *__restrict__ in_stage,
     const float2 *__restrict__ ref_now, const float2 *__restrict__ ref_prev,
-    const float2 *__restrict__ grad_prev, double *partials, const DevScalars *sc) {
+    const float2 *__restrict__ grad_prev, double *partials, float2 *pos_out, float2 *gtot_out,
+    const DevScalars *sc) {
   if (sc->done) return;
   int o = xcd_block() * blockDim.x + threadIdx.x;
   double ln = 0, ld = 0, nw = 0, nd = 0;
@@ -453,6 +454,9 @@ __global__ void __launch_bounds__(kBlock) gather_combine_kernel(
       t.y = pre * (lam * d.y - wl.y);
     }
     gtot[o] = t;
+    // the optimiser's copies of y_k and g_k for the next trials
+    if (pos_out) pos_out[o] = pos[o];
+    if (gtot_out) gtot_out[o] = t;
     if (lipschitz && in_stage[o]) {
       float2 gp = grad_prev[o];
       float2 rn = ref_now ? ref_now[o] : pos[o];
@@ -618,17 +622,17 @@ void launch_field_gather(Ctx &c) {
 }
 
 void launch_combine(Ctx &c, bool recompute, bool lipschitz, const float2 *ref_now, const float2 *ref_prev,
-                    const float2 *grad_prev) {
+                    const float2 *grad_prev, float2 *pos_out, float2 *gtot_out) {
   hipLaunchKernelGGL(gather_combine_kernel, dim3(object_blocks(c.O)), dim3(kBlock), 0, c.stream, c.O, c.N,
                      c.pos, c.kind, c.npins, c.area, c.node_pin_start, c.node_pin_lane, c.pin_grad, c.gd, c.gwl,
                      c.gtot, recompute ? 1 : 0, lipschitz ? 1 : 0, c.in_stage, ref_now, ref_prev, grad_prev,
-                     c.partials, c.sc);
+                     c.partials, pos_out, gtot_out, c.sc);
 }
 
 void launch_gather_combine(Ctx &c, bool recompute, bool lipschitz, const float2 *ref_now,
-                           const float2 *ref_prev, const float2 *grad_prev) {
+                           const float2 *ref_prev, const float2 *grad_prev, float2 *pos_out, float2 *gtot_out) {
   if (recompute) launch_field_gather(c);
-  launch_combine(c, recompute, lipschitz, ref_now, ref_prev, grad_prev);
+  launch_combine(c, recompute, lipschitz, ref_now, ref_prev, grad_prev, pos_out, gtot_out);
 }
 
 void launch_lambda_init(Ctx &c) {

@@ -754,12 +754,16 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
     } else {
       // combine-only re-evaluation at y_k with the updated lambda (the
       // positions were evaluated by the accepted trial or the caller)
-      if (k == 0 && !c.eval_valid) evaluate(c, false, true, nullptr, c.last_ref, c.last_grad, false);
-      else launch_gather_combine(c, false, true, nullptr, c.last_ref, c.last_grad);
+      // (y_k and g_k are also copied for the trials: cur_ref, cur_grad)
+      if (k == 0 && !c.eval_valid) {
+        evaluate(c, false, true, nullptr, c.last_ref, c.last_grad, false);
+        launch_positions_to_ref(c, c.cur_ref);
+        launch_copy_float2(c, c.cur_grad, c.gtot, c.O);
+      } else {
+        launch_gather_combine(c, false, true, nullptr, c.last_ref, c.last_grad, c.cur_ref, c.cur_grad);
+      }
       launch_step_scalar(c, 1);
       if (features) launch_ctrl_step(c, ctrl, learned ? 1 : 0);
-      launch_positions_to_ref(c, c.cur_ref);
-      launch_copy_float2(c, c.cur_grad, c.gtot, c.O);
       int tries = 0;
       bool stopped = false;
       while (true) {

@@ -250,12 +250,15 @@ struct Ctx {
 // ---- launchers (density.hip) ----
 void launch_scatter(Ctx &c);
 void launch_finalize_density(Ctx &c);
+// pos_out / gtot_out (optional): copies of the positions and of the total
+// gradient written by the same pass
 void launch_gather_combine(Ctx &c, bool recompute_gather, bool lipschitz, const float2 *ref_now,
-                           const float2 *ref_prev, const float2 *grad_prev);
+                           const float2 *ref_prev, const float2 *grad_prev, float2 *pos_out = nullptr,
+                           float2 *gtot_out = nullptr);
 void launch_field_gather(Ctx &c);
 void launch_node_wl(Ctx &c, hipStream_t st);
 void launch_combine(Ctx &c, bool recompute, bool lipschitz, const float2 *ref_now, const float2 *ref_prev,
-                    const float2 *grad_prev);
+                    const float2 *grad_prev, float2 *pos_out = nullptr, float2 *gtot_out = nullptr);
 bool tile_sort_create(Ctx &c, std::string &err);
 void tile_sort_destroy(Ctx &c);
 // ---- wirelength.hip ----
