@@ -126,14 +126,19 @@ __global__ void __launch_bounds__(kSortThreads) tile_hist_kernel(
   }
 }
 
-// Pass 2 (one block): key totals -> first slot of each key (the totals are
-// cleared for the next sort), and chunks of <= kTileChunk objects per key
-// (the sentinel key makes none).
-__global__ void __launch_bounds__(kSortThreads) tile_keyscan_kernel(int nkeys, uint32_t *ktotal, uint32_t *kstart,
-                                                                     int4 *chunks, int *nchunks) {
+// Pass 2: every block scans the key totals into the first slot of each key
+// (the same result in every block) and places its objects at that slot + its
+// offset inside the key + an LDS cursor.  Block 0 also writes the chunks of
+// <= kTileChunk objects per key (the sentinel key makes none).  The totals
+// are cleared by the scatter, after every block has read them.
+__global__ void __launch_bounds__(kSortThreads) tile_place_kernel(
+    int O, int per, int nkeys, const uint32_t *__restrict__ okey, const uint32_t *__restrict__ bcount,
+    const uint32_t *__restrict__ ktotal, const float2 *__restrict__ pos, const float *__restrict__ w,
+    const float *__restrict__ h, uint32_t *sobj, float4 *svals, int4 *chunks, int *nchunks) {
+  extern __shared__ uint32_t cursor[];
   __shared__ int pslot[kSortThreads], pchunk[kSortThreads];
-  const int per = (nkeys + blockDim.x - 1) / blockDim.x;
-  const int k0 = threadIdx.x * per, k1 = min(k0 + per, nkeys);
+  const int kper = (nkeys + blockDim.x - 1) / blockDim.x;
+  const int k0 = threadIdx.x * kper, k1 = min(k0 + kper, nkeys);
   int slots = 0, cnt = 0;
   for (int k = k0; k < k1; ++k) {
     const int n = (int)ktotal[k];
@@ -151,34 +156,23 @@ __global__ void __launch_bounds__(kSortThreads) tile_keyscan_kernel(int nkeys, u
     pchunk[threadIdx.x] += b;
     __syncthreads();
   }
-  int slot = pslot[threadIdx.x] - slots, pos = pchunk[threadIdx.x] - cnt;
+  int slot = pslot[threadIdx.x] - slots, cpos = pchunk[threadIdx.x] - cnt;
+  const uint32_t *bc = bcount + (size_t)blockIdx.x * nkeys;
   for (int k = k0; k < k1; ++k) {
     const int n = (int)ktotal[k];
-    ktotal[k] = 0;
-    kstart[k] = (uint32_t)slot;
-    if (k < nkeys - 1)
-      for (int c = 0; c < n; c += kTileChunk) chunks[pos++] = make_int4(k, slot + c, slot + min(c + kTileChunk, n), 0);
+    cursor[k] = (uint32_t)slot + bc[k];
+    if (blockIdx.x == 0 && k < nkeys - 1)
+      for (int c = 0; c < n; c += kTileChunk) chunks[cpos++] = make_int4(k, slot + c, slot + min(c + kTileChunk, n), 0);
     slot += n;
   }
-  if (threadIdx.x == blockDim.x - 1) *nchunks = pchunk[threadIdx.x];
-}
-
-// Pass 3: block b places its objects at kstart[key] + its offset inside the
-// key + an LDS cursor.
-__global__ void __launch_bounds__(kSortThreads) tile_place_kernel(
-    int O, int per, int nkeys, const uint32_t *__restrict__ okey, const uint32_t *__restrict__ bcount,
-    const uint32_t *__restrict__ kstart, const float2 *__restrict__ pos, const float *__restrict__ w,
-    const float *__restrict__ h, uint32_t *sobj, float4 *svals) {
-  extern __shared__ uint32_t cursor[];
-  for (int k = threadIdx.x; k < nkeys; k += blockDim.x)
-    cursor[k] = kstart[k] + bcount[(size_t)blockIdx.x * nkeys + k];
+  if (blockIdx.x == 0 && threadIdx.x == blockDim.x - 1) *nchunks = pchunk[threadIdx.x];
   __syncthreads();
   const int o0 = blockIdx.x * per, o1 = min(O, o0 + per);
   for (int o = o0 + threadIdx.x; o < o1; o += blockDim.x) {
-    const uint32_t slot = atomicAdd(&cursor[okey[o]], 1u);
+    const uint32_t sl = atomicAdd(&cursor[okey[o]], 1u);
     const float2 p = pos[o];
-    sobj[slot] = (uint32_t)o;
-    svals[slot] = make_float4(p.x, p.y, w[o], h[o]);
+    sobj[sl] = (uint32_t)o;
+    svals[sl] = make_float4(p.x, p.y, w[o], h[o]);
   }
 }
 
@@ -190,8 +184,11 @@ __global__ void __launch_bounds__(kBlock) tile_scatter_kernel(const int4 *__rest
                                                                const float4 *__restrict__ vals, Grid g, int tyc,
                                                                int ntiles,
                                                                double scale, unsigned long long *node_q,
-                                                               unsigned long long *filler_q) {
+                                                               unsigned long long *filler_q, int nkeys,
+                                                               uint32_t *ktotal) {
   __shared__ unsigned long long win[kTileWin * kTileWin];
+  // the key totals of this sort are read; zero them for the next one
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nkeys; k += gridDim.x * blockDim.x) ktotal[k] = 0;
   const int total = *nchunks;
   for (int ci = blockIdx.x; ci < total; ci += gridDim.x) {
     const int4 ch = chunks[ci];
@@ -528,12 +525,10 @@ void launch_scatter(Ctx &c) {
   hipLaunchKernelGGL(tile_hist_kernel, dim3(t.nblocks), dim3(kSortThreads), lds, c.stream, c.O, t.per, c.pos, c.w,
                      c.h, c.kind, c.is_big, g, t.tyc, t.ntiles, t.nkeys, t.okey, t.bcount, t.ktotal, nb, c.node_q,
                      c.filler_q);
-  hipLaunchKernelGGL(tile_keyscan_kernel, dim3(1), dim3(kSortThreads), 0, c.stream, t.nkeys, t.ktotal, t.kstart,
-                     t.chunks, t.nchunks);
   hipLaunchKernelGGL(tile_place_kernel, dim3(t.nblocks), dim3(kSortThreads), lds, c.stream, c.O, t.per, t.nkeys,
-                     t.okey, t.bcount, t.kstart, c.pos, c.w, c.h, t.sobj, t.svals);
+                     t.okey, t.bcount, t.ktotal, c.pos, c.w, c.h, t.sobj, t.svals, t.chunks, t.nchunks);
   hipLaunchKernelGGL(tile_scatter_kernel, dim3(t.grid), dim3(kBlock), 0, c.stream, t.chunks, t.nchunks, t.svals, g,
-                     t.tyc, t.ntiles, scale, c.node_q, c.filler_q);
+                     t.tyc, t.ntiles, scale, c.node_q, c.filler_q, t.nkeys, t.ktotal);
   if (c.num_big)
     hipLaunchKernelGGL(scatter_big_kernel, dim3(c.num_big), dim3(kBlock), 0, c.stream, c.big, c.pos, c.w,
                        c.h, c.kind, g, scale, c.node_q, c.filler_q);
@@ -557,7 +552,6 @@ bool tile_sort_create(Ctx &c, std::string &err) {
   bool ok = hipMalloc(&t.okey, (size_t)O * 4) == hipSuccess &&
             hipMalloc(&t.bcount, (size_t)t.nkeys * t.nblocks * 4) == hipSuccess &&
             hipMalloc(&t.ktotal, (size_t)t.nkeys * 4) == hipSuccess &&
-            hipMalloc(&t.kstart, (size_t)t.nkeys * 4) == hipSuccess &&
             hipMalloc(&t.sobj, (size_t)O * 4) == hipSuccess && hipMalloc(&t.svals, (size_t)O * 16) == hipSuccess &&
             hipMalloc(&t.chunks, max_chunks * sizeof(int4)) == hipSuccess &&
             hipMalloc(&t.nchunks, sizeof(int)) == hipSuccess;
@@ -569,7 +563,7 @@ bool tile_sort_create(Ctx &c, std::string &err) {
 
 void tile_sort_destroy(Ctx &c) {
   TileSort &t = c.tiles;
-  void *p[] = {t.okey, t.bcount, t.ktotal, t.kstart, t.sobj, t.svals, t.chunks, t.nchunks};
+  void *p[] = {t.okey, t.bcount, t.ktotal, t.sobj, t.svals, t.chunks, t.nchunks};
   for (void *q : p)
     if (q) (void)hipFree(q);
   t = TileSort();

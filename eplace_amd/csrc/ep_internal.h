@@ -126,7 +126,6 @@ struct TileSort {
   uint32_t *okey = nullptr;    // [O] key of each object
   uint32_t *bcount = nullptr;  // [nblocks][nkeys] offset of each block's run inside the key
   uint32_t *ktotal = nullptr;  // [nkeys] running key totals, zero between sorts
-  uint32_t *kstart = nullptr;  // [nkeys] first sorted slot of each key
   uint32_t *sobj = nullptr;    // [O] objects in key order
   float4 *svals = nullptr;     // [O] their (x, y, w, h)
   int4 *chunks = nullptr;      // (key, begin, end) runs of <= kTileChunk objects of one key
