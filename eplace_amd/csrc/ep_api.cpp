@@ -158,7 +158,8 @@ int create(Ctx &c, const ep_design *d, const ep_params *p) {
   EP_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
   EP_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
   for (auto &e : c.ev) EP_HIP(hipEventCreate(&e));
-  EP_HIP(hipHostMalloc((void **)&c.h_sc, sizeof(DevScalars), hipHostMallocDefault));
+  EP_HIP(hipHostMalloc((void **)&c.h_sc, sizeof(Ctx::HostScalars), hipHostMallocCoherent));
+  c.h_sc->seq = 0;
   c.td = p->target_density;
   c.wl_model = p->wirelength_model;
   c.exp_mode = p->exp_mode;
@@ -616,11 +617,24 @@ int sync(Ctx &c) {
   return EP_OK;
 }
 
+// The scalars once the stream reaches this point: the host polls the
+// published sequence number; after 20 ms of polling it waits on the stream
+// instead, which also reports a failed kernel.
 int read_scalars(Ctx &c, DevScalars &h) {
-  EP_HIP(hipMemcpyAsync(c.h_sc, c.sc, sizeof(h), hipMemcpyDeviceToHost, c.stream));
-  int rc = sync(c);
-  if (rc) return rc;
-  h = *c.h_sc;
+  const unsigned int seq = ++c.pub_seq;
+  launch_publish_scalars(c, seq);
+  EP_HIP(hipGetLastError());
+  const auto t0 = std::chrono::steady_clock::now();
+  for (unsigned int n = 1; __atomic_load_n(&c.h_sc->seq, __ATOMIC_ACQUIRE) != seq; ++n) {
+    if ((n & 1023) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20)) {
+      int rc = sync(c);
+      if (rc) return rc;
+      if (__atomic_load_n(&c.h_sc->seq, __ATOMIC_ACQUIRE) != seq)
+        return c.err = "scalars were not published", EP_ERR_HIP;
+      break;
+    }
+  }
+  h = c.h_sc->s;
   return EP_OK;
 }
 

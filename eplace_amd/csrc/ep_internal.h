@@ -141,9 +141,15 @@ struct Ctx {
   // join before the combine)
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  // pinned host copy of the scalars: a read is one copy queued behind the
-  // stream's work and one wait
-  DevScalars *h_sc = nullptr;
+  // Pinned coherent host copy of the scalars and its sequence number: a
+  // read queues a kernel that stores the scalars there and then the number,
+  // and the host polls the number (no copy engine, no stream wait).
+  struct HostScalars {
+    DevScalars s;
+    unsigned int seq;
+  };
+  HostScalars *h_sc = nullptr;
+  unsigned int pub_seq = 0;
   // per-evaluation phase events: 0 start, 1 scatter end, 2 finalise end,
   // 3 Poisson end, 4 field gather end, 5 combine end (stream), 6 wiring
   // start, 8 wiring kernels end, 7 node sums end (stream2)
@@ -279,6 +285,7 @@ void launch_set_positions(Ctx &c, const float2 *xy_stage, const int32_t *stage_i
 void launch_copy_float2(Ctx &c, float2 *dst, const float2 *src, int64_t n);
 void launch_write_main_back(Ctx &c, const float2 *main_sol);
 void launch_done_check(Ctx &c, int next_iter);
+void launch_publish_scalars(Ctx &c, unsigned int seq);
 void launch_done_check_ex(Ctx &c, int next_iter, double *log, int log_cap, int set_iter);
 void launch_nesterov_move_ex(Ctx &c, const float2 *ref, const float2 *grad, const float2 *main_in,
                              float2 *main_out, float2 *ref_unclamped_out, float2 *save_ref, float2 *save_grad);

@@ -200,6 +200,24 @@ void launch_done_check_ex(Ctx &c, int next_iter, double *log, int log_cap, int s
   hipLaunchKernelGGL(done_check_kernel, dim3(1), dim3(1), 0, c.stream, c.sc, next_iter, log, log_cap, set_iter);
 }
 
+// The scalars to pinned coherent host memory, then the sequence number with
+// a system-scope release: once the host sees the number, the scalars it
+// reads are this point's.
+__global__ void __launch_bounds__(64) publish_scalars_kernel(const DevScalars *sc, Ctx::HostScalars *host,
+                                                             unsigned int seq) {
+  static_assert(sizeof(DevScalars) % 4 == 0, "scalars are copied as 32-bit words");
+  const uint32_t *src = reinterpret_cast<const uint32_t *>(sc);
+  uint32_t *dst = reinterpret_cast<uint32_t *>(&host->s);
+  for (int i = threadIdx.x; i < (int)(sizeof(DevScalars) / 4); i += blockDim.x) dst[i] = src[i];
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(&host->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+void launch_publish_scalars(Ctx &c, unsigned int seq) {
+  hipLaunchKernelGGL(publish_scalars_kernel, dim3(1), dim3(64), 0, c.stream, c.sc, c.h_sc, seq);
+}
+
 void launch_done_check(Ctx &c, int next_iter) { launch_done_check_ex(c, next_iter, nullptr, 0, 1); }
 
 
