@@ -591,7 +591,23 @@ __global__ void __launch_bounds__(kBlock) node_wl_kernel(int N, const int32_t *_
   const int o = xcd_block() * blockDim.x + threadIdx.x;
   if (o >= N) return;
   float2 wl = make_float2(0.f, 0.f);
-  for (int q = node_pin_start[o], e = node_pin_start[o + 1]; q < e; ++q) {
+  int q = node_pin_start[o];
+  const int e = node_pin_start[o + 1];
+  // four gathers in flight, summed in pin order
+  for (; q + 4 <= e; q += 4) {
+    const int l0 = node_pin_lane[q], l1 = node_pin_lane[q + 1], l2 = node_pin_lane[q + 2],
+              l3 = node_pin_lane[q + 3];
+    const float2 p0 = pin_grad[l0], p1 = pin_grad[l1], p2 = pin_grad[l2], p3 = pin_grad[l3];
+    wl.x += p0.x;
+    wl.y += p0.y;
+    wl.x += p1.x;
+    wl.y += p1.y;
+    wl.x += p2.x;
+    wl.y += p2.y;
+    wl.x += p3.x;
+    wl.y += p3.y;
+  }
+  for (; q < e; ++q) {
     const float2 pg = pin_grad[node_pin_lane[q]];
     wl.x += pg.x;
     wl.y += pg.y;
