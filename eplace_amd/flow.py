@@ -25,10 +25,11 @@ from .qplace import quadratic_placement
 
 def place(d: Design, target_density: float = 1.0, target_overflow: float = 0.10, *, qp: bool = True,
           qp_iterations: int = 20, mgp: bool = True, mlg: bool = True, cgp: bool = True, legal: bool = False,
-          method: str = "vanilla", bins=0, wirelength: str = "WA", poisson_mode: int = 0, device: int = 0,
-          verbose: bool = False) -> dict:
+          method: str = "vanilla", controller=None, bins=0, wirelength: str = "WA", poisson_mode: int = 0,
+          device: int = 0, verbose: bool = False) -> dict:
     """Runs the flow on d and returns per-stage results; d.node_x/node_y hold
-    the final node centres (the legal ones when legal=True)."""
+    the final node centres (the legal ones when legal=True).  method
+    "learned" takes its step / lambda controller from `controller`."""
     say = print if verbose else (lambda *a, **k: None)
     out: dict = {}
     t0 = time.perf_counter()
@@ -39,7 +40,8 @@ def place(d: Design, target_density: float = 1.0, target_overflow: float = 0.10,
     try:
         gp.initialization()
         if mgp:
-            out["mgp"] = EplaceNesterovOpt(gp, method=method, target_overflow=target_overflow, log=False).opt()
+            out["mgp"] = EplaceNesterovOpt(gp, method=method, controller=controller, target_overflow=target_overflow,
+                                           log=False).opt()
             say(f"mGP: {out['mgp']['iterations']} iterations, HPWL {out['mgp']['final_hpwl']:.6g}")
         macros = int(d.node_is_macro().sum())
         if macros > 0 and mlg:
@@ -49,9 +51,9 @@ def place(d: Design, target_density: float = 1.0, target_overflow: float = 0.10,
             say(f"mLG: {macros} macros, overlap {out['mlg']['final_overlap']}, HPWL {out['mlg']['final_hpwl']:.6g}")
             if cgp:
                 gp.switch2FillerOnly()
-                out["fillers"] = EplaceNesterovOpt(gp, method=method, log=False).opt()
+                out["fillers"] = EplaceNesterovOpt(gp, method=method, controller=controller, log=False).opt()
                 gp.switch2cGP()
-                out["cgp"] = EplaceNesterovOpt(gp, method=method, log=False).opt()
+                out["cgp"] = EplaceNesterovOpt(gp, method=method, controller=controller, log=False).opt()
                 say(f"cGP: {out['cgp']['iterations']} iterations, HPWL {out['cgp']['final_hpwl']:.6g}")
         xy = gp.read("positions")[: d.num_nodes].copy()
         out["hpwl"] = gp.calcHPWL()
