@@ -309,7 +309,8 @@ def main():
             device_sync()
             t0 = time.perf_counter()
             gp.initialization()
-            rl = EplaceNesterovOpt(gp, method="learned", controller=ctrl, target_overflow=0.10).opt()
+            rl = EplaceNesterovOpt(gp, method="learned", controller=ctrl, target_overflow=0.10,
+                                   log=False).opt()
             device_sync()
             tl = time.perf_counter() - t0
             conv["learned"] = {"seconds_to_overflow_0.10": tl, "iterations": rl["iterations"],
