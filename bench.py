@@ -88,6 +88,37 @@ PHASE_KERNELS = {
 }
 
 
+def copy_peak_gbs(mib=1024, reps=10):
+    """Bandwidth of a plain device-to-device copy (read + write bytes per
+    second), timed with HIP events on the null stream: the practical HBM
+    ceiling next to the datasheet peak."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = mib << 20
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    ms = ctypes.c_float()
+    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)) == 0
+    try:
+        hip.hipMemset(src, 1, ctypes.c_size_t(nbytes))
+        hip.hipEventCreate(ctypes.byref(a))
+        hip.hipEventCreate(ctypes.byref(b))
+        kind = 3  # hipMemcpyDeviceToDevice
+        hip.hipMemcpyAsync(dst, src, ctypes.c_size_t(nbytes), kind, None)
+        hip.hipEventRecord(a, None)
+        for _ in range(reps):
+            hip.hipMemcpyAsync(dst, src, ctypes.c_size_t(nbytes), kind, None)
+        hip.hipEventRecord(b, None)
+        hip.hipEventSynchronize(b)
+        hip.hipEventElapsedTime(ctypes.byref(ms), a, b)
+        hip.hipEventDestroy(a)
+        hip.hipEventDestroy(b)
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
+    return 2.0 * nbytes / (ms.value * 1e-3 / reps) / 1e9
+
+
 def phase_traffic(kernel_key, info):
     """HBM bytes of one evaluation's phase (all its launches) from the
     committed rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE scaled by a
@@ -348,6 +379,13 @@ def main():
                       f"({qp['speedup_per_outer_iteration']:.0f}x)", file=sys.stderr, flush=True)
         del dq
 
+    copy_gbs = copy_peak_gbs()
+    print(f"[bench] device copy {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    for v in [roof] + list(roof_all.values()):
+        if v["bound"] == "hbm":
+            v["copy_peak"] = copy_gbs
+            v["frac_of_copy"] = v["achieved"] / copy_gbs
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(d, args.method, args.cpu_iters, init_pos)
@@ -375,7 +413,8 @@ def main():
                    "bins": [info["bins_x"], info["bins_y"]]},
         "roofline": roof,
         "roofline_phases": {k: {"bound": v["bound"], "achieved": v["achieved"], "unit": v["unit"],
-                                "frac": v["frac"], "traffic": v["traffic"]} for k, v in roof_all.items()},
+                                "frac": v["frac"], "frac_of_copy": v.get("frac_of_copy"), "traffic": v["traffic"]}
+                            for k, v in roof_all.items()},
         "phase_ms_per_evaluation": {k: v * 1e3 for k, v in ph.items()},
         "converge": conv,
         "qp": qp,
