@@ -277,14 +277,10 @@ void launch_poisson(Ctx &c, bool with_phi);
 // ---- nesterov.hip ----
 void launch_positions_to_ref(Ctx &c, float2 *dst);
 void launch_step_scalar(Ctx &c, int bktrk_first);
-void launch_nesterov_move(Ctx &c, const float2 *ref, const float2 *grad, const float2 *main_in,
-                          float2 *main_out, float2 *ref_unclamped_out);
 void launch_accept(Ctx &c);
 void launch_lambda_init(Ctx &c);
 void launch_set_positions(Ctx &c, const float2 *xy_stage, const int32_t *stage_idx, int64_t n);
 void launch_copy_float2(Ctx &c, float2 *dst, const float2 *src, int64_t n);
-void launch_write_main_back(Ctx &c, const float2 *main_sol);
-void launch_done_check(Ctx &c, int next_iter);
 void launch_publish_scalars(Ctx &c, unsigned int seq);
 void launch_done_check_ex(Ctx &c, int next_iter, double *log, int log_cap, int set_iter);
 void launch_nesterov_move_ex(Ctx &c, const float2 *ref, const float2 *grad, const float2 *main_in,

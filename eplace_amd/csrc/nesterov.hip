@@ -191,11 +191,6 @@ void launch_nesterov_move_ex(Ctx &c, const float2 *ref, const float2 *grad, cons
                      core_of(c), c.sc);
 }
 
-void launch_nesterov_move(Ctx &c, const float2 *ref, const float2 *grad, const float2 *main_in,
-                          float2 *main_out, float2 *ref_unclamped_out) {
-  launch_nesterov_move_ex(c, ref, grad, main_in, main_out, ref_unclamped_out, nullptr, nullptr);
-}
-
 void launch_done_check_ex(Ctx &c, int next_iter, double *log, int log_cap, int set_iter) {
   hipLaunchKernelGGL(done_check_kernel, dim3(1), dim3(1), 0, c.stream, c.sc, next_iter, log, log_cap, set_iter);
 }
@@ -218,7 +213,6 @@ void launch_publish_scalars(Ctx &c, unsigned int seq) {
   hipLaunchKernelGGL(publish_scalars_kernel, dim3(1), dim3(64), 0, c.stream, c.sc, c.h_sc, seq);
 }
 
-void launch_done_check(Ctx &c, int next_iter) { launch_done_check_ex(c, next_iter, nullptr, 0, 1); }
 
 
 }  // namespace ep
