@@ -80,35 +80,94 @@ __device__ __forceinline__ void stage_weights(const float *__restrict__ P, float
 
 __device__ __forceinline__ float flog(double v) { return isfinite(v) ? (float)v : 0.f; }
 
+// The rows of the network that lane t needs (row t of W1 and W2, row t of W3
+// for t < NO, and their biases), loaded into registers at the start of the
+// step kernel so that their latency overlaps the feature computation.
+struct LaneWeights {
+  float4 w1[F / 4], w2[H / 4], w3[H / 4];
+  float b1, b2, b3;
+};
+
+__device__ __forceinline__ void load_lane_weights(const float *__restrict__ P, LaneWeights &r) {
+  static_assert(F % 4 == 0 && H % 4 == 0 && oW2 % 4 == 0 && oW3 % 4 == 0, "float4 rows");
+  const int t = threadIdx.x, t3 = t < NO ? t : 0;
+  const float4 *w1 = reinterpret_cast<const float4 *>(P + oW1 + t * F);
+  const float4 *w2 = reinterpret_cast<const float4 *>(P + oW2 + t * H);
+  const float4 *w3 = reinterpret_cast<const float4 *>(P + oW3 + t3 * H);
+#pragma unroll
+  for (int q = 0; q < F / 4; ++q) r.w1[q] = w1[q];
+#pragma unroll
+  for (int q = 0; q < H / 4; ++q) r.w2[q] = w2[q];
+#pragma unroll
+  for (int q = 0; q < H / 4; ++q) r.w3[q] = w3[q];
+  r.b1 = P[ob1 + t];
+  r.b2 = P[ob2 + t];
+  r.b3 = P[ob3 + t3];
+}
+
+// a += w . v over 4*n terms, in index order (the order of mlp_forward).
+template <int n>
+__device__ __forceinline__ float dot_rows(float a, const float4 (&w)[n], const float *v) {
+#pragma unroll
+  for (int q = 0; q < n; ++q) {
+    a += w[q].x * v[4 * q];
+    a += w[q].y * v[4 * q + 1];
+    a += w[q].z * v[4 * q + 2];
+    a += w[q].w * v[4 * q + 3];
+  }
+  return a;
+}
+
+// mlp_forward with the weights in registers: the same sums in the same order.
+__device__ __forceinline__ void mlp_forward_regs(const LaneWeights &r, const float *x, float *h1, float *h2,
+                                                 float *out) {
+  const int t = threadIdx.x;
+  h1[t] = tanhf(dot_rows(r.b1, r.w1, x));
+  __syncthreads();
+  h2[t] = tanhf(dot_rows(r.b2, r.w2, h1));
+  __syncthreads();
+  if (t < NO) out[t] = dot_rows(r.b3, r.w3, h2);
+  __syncthreads();
+}
+
 // Features of iteration k and, when active, the controller's step and lambda
-// multiplier.  One block of H threads.
+// multiplier.  One block of H threads; lane i < F computes feature i (the
+// logarithms of the ratios in parallel).
 __global__ void __launch_bounds__(H) ctrl_step_kernel(DevScalars *sc, const float *__restrict__ P, int active) {
+  LaneWeights r;
+  if (active) load_lane_weights(P, r);
   if (sc->done) return;
-  __shared__ float W[NP];
   __shared__ float x[F], h1[H], h2[H], out[NO];
-  if (threadIdx.x == 0) {
-    const int k = sc->iter;
+  if (threadIdx.x < F) {
+    const int i = threadIdx.x, k = sc->iter;
     const double tau = sc->overflow, lam = sc->lambda, hp = sc->hpwl, prop = sc->step;
-    float f[F];
-    f[0] = (float)tau;
-    f[1] = k > 0 ? (float)(tau - sc->tau_prev) : 0.f;
-    f[2] = flog(log(lam / sc->lambda0));
-    f[3] = flog(log(hp / sc->hpwl0));
-    f[4] = k > 0 ? flog(100.0 * (hp - sc->hpwl_prev) / hp) : 0.f;
-    f[5] = flog(log(sqrt(sc->gwl_norm2) / (lam * sqrt(sc->gd_norm2))));
-    f[6] = k >= 1 ? flog(log(sc->step_hist[0] / sc->step0)) : 0.f;
-    f[7] = k >= 2 ? flog(log(sc->step_hist[0] / sc->step_hist[1])) : 0.f;
-    f[8] = k >= 3 ? flog(log(sc->step_hist[1] / sc->step_hist[2])) : 0.f;
-    f[9] = (float)k / 1000.f;
-    f[10] = k >= 1 ? flog(log(prop / sc->step_hist[0])) : 0.f;
-    f[11] = k > 0 ? flog(log(lam / sc->lambda_before)) : 0.f;
-    for (int i = 0; i < F; ++i) {
-      sc->feat[i] = f[i];
-      x[i] = f[i];
+    double num = 1.0, den = 1.0;  // feature = log(num / den) when `ratio`
+    bool ratio = true, have = true;
+    float direct = 0.f;
+    switch (i) {
+      case 0: ratio = false; direct = (float)tau; break;
+      case 1: ratio = false; direct = k > 0 ? (float)(tau - sc->tau_prev) : 0.f; break;
+      case 2: num = lam; den = sc->lambda0; break;
+      case 3: num = hp; den = sc->hpwl0; break;
+      case 4: ratio = false; direct = k > 0 ? flog(100.0 * (hp - sc->hpwl_prev) / hp) : 0.f; break;
+      case 5: num = sqrt(sc->gwl_norm2); den = lam * sqrt(sc->gd_norm2); break;
+      case 6: have = k >= 1; num = sc->step_hist[0]; den = sc->step0; break;
+      case 7: have = k >= 2; num = sc->step_hist[0]; den = sc->step_hist[1]; break;
+      case 8: have = k >= 3; num = sc->step_hist[1]; den = sc->step_hist[2]; break;
+      case 9: ratio = false; direct = (float)k / 1000.f; break;
+      case 10: have = k >= 1; num = prop; den = sc->step_hist[0]; break;
+      default: have = k > 0; num = lam; den = sc->lambda_before; break;
     }
+    const float f = ratio ? (have ? flog(log(num / den)) : 0.f) : direct;
+    sc->feat[i] = f;
+    x[i] = active ? (f - P[oXmu + i]) / P[oXsd + i] : f;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double lam = sc->lambda, hp = sc->hpwl, prop = sc->step;
     sc->step_prop = prop;
     sc->lambda_before = lam;
-    sc->tau_prev = tau;
+    sc->tau_prev = sc->overflow;
     sc->hpwl_prev = hp;
     sc->backtracks_mark = sc->backtracks;
     sc->trial = 0;
@@ -116,14 +175,10 @@ __global__ void __launch_bounds__(H) ctrl_step_kernel(DevScalars *sc, const floa
     sc->lam_mult = 1.0;
     sc->step_used = prop;
   }
-  __syncthreads();
   if (!active) return;
-  stage_weights(P, W);
-  if (threadIdx.x < F) x[threadIdx.x] = (x[threadIdx.x] - W[oXmu + threadIdx.x]) / W[oXsd + threadIdx.x];
-  __syncthreads();
-  mlp_forward(W, x, h1, h2, out);
+  mlp_forward_regs(r, x, h1, h2, out);
   if (threadIdx.x == 0) {
-    const float o0 = out[0] * W[oYsd] + W[oYmu], o1 = out[1] * W[oYsd + 1] + W[oYmu + 1];
+    const float o0 = out[0] * P[oYsd] + P[oYmu], o1 = out[1] * P[oYsd + 1] + P[oYmu + 1];
     const double s = fmin(fmax((double)o0, -1.3862943611198906), 0.0);
     const double l = fmin(fmax((double)o1, -0.05), 0.1);
     if (isfinite(s) && isfinite(l)) {
