@@ -286,8 +286,10 @@ __global__ void __launch_bounds__(kFinish) finish_density_kernel(int nblocks, in
                                                                  double node_area_scaled, const double *partials,
                                                                  DevScalars *sc) {
   if (sc->done) return;
-  double tot = block_sum_partials(partials + kPartialTau, nblocks);
-  double rtot = block_sum_partials(partials + kPartialRho, nblocks);
+  const double *const src[2] = {partials + kPartialTau, partials + kPartialRho};
+  double sum[2];
+  block_sum_partials_strided_n(src, nblocks, 1, sum);
+  const double tot = sum[0], rtot = sum[1];
   if (threadIdx.x == 0) {
     double tau = tot / node_area_scaled;
     sc->overflow = tau;

@@ -43,10 +43,11 @@ __global__ void copy_float2_kernel(int64_t n, float2 *dst, const float2 *src) {
 // counted in bad_numeric.
 __global__ void __launch_bounds__(kFinish) step_scalar_kernel(int nblocks, double *partials, DevScalars *sc) {
   if (sc->done) return;
-  double num = block_sum_partials_strided(partials + kPartialLip, nblocks, 2);
-  double den = block_sum_partials_strided(partials + kPartialLip + 1, nblocks, 2);
-  double nw = block_sum_partials_strided(partials + kPartialNorm, nblocks, 2);
-  double nd = block_sum_partials_strided(partials + kPartialNorm + 1, nblocks, 2);
+  const double *const src[4] = {partials + kPartialLip, partials + kPartialLip + 1, partials + kPartialNorm,
+                                partials + kPartialNorm + 1};
+  double sum[4];
+  block_sum_partials_strided_n(src, nblocks, 2, sum);
+  const double num = sum[0], den = sum[1], nw = sum[2], nd = sum[3];
   if (threadIdx.x == 0) {
     sc->gwl_norm2 = nw;
     sc->gd_norm2 = nd;
@@ -73,8 +74,10 @@ __global__ void __launch_bounds__(kFinish) step_scalar_kernel(int nblocks, doubl
 // 0.95 * step <= 1/L', L' from the trial evaluation; step := 1/L' either way.
 __global__ void __launch_bounds__(kFinish) accept_kernel(int nblocks, double *partials, DevScalars *sc) {
   if (sc->done) return;
-  double num = block_sum_partials_strided(partials + kPartialLip, nblocks, 2);
-  double den = block_sum_partials_strided(partials + kPartialLip + 1, nblocks, 2);
+  const double *const src[2] = {partials + kPartialLip, partials + kPartialLip + 1};
+  double sum[2];
+  block_sum_partials_strided_n(src, nblocks, 2, sum);
+  const double num = sum[0], den = sum[1];
   if (threadIdx.x == 0) {
     double ns = 1.0 / (sqrt(num) / sqrt(den));
     if (!isfinite(ns) || ns <= 0.0) {

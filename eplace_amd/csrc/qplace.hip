@@ -371,8 +371,10 @@ __global__ void __launch_bounds__(kQpThreads) qp_residual_kernel(QpSys S, QpSolv
 __global__ void __launch_bounds__(kFinish) qp_setup_finish_kernel(QpSolveScalars *sc, const double *partials,
                                                                    int nblocks, float tol, int max_iter) {
   const int d = blockIdx.x;
-  const float rr = (float)block_sum_partials(partials + (2 * d) * kQpMaxBlocks, nblocks);
-  const float bb = (float)block_sum_partials(partials + (2 * d + 1) * kQpMaxBlocks, nblocks);
+  const double *const src[2] = {partials + (2 * d) * kQpMaxBlocks, partials + (2 * d + 1) * kQpMaxBlocks};
+  double sum[2];
+  block_sum_partials_strided_n(src, nblocks, 1, sum);
+  const float rr = (float)sum[0], bb = (float)sum[1];
   if (threadIdx.x) return;
   QpSolveScalars s = QpSolveScalars{};
   s.max_iter = max_iter;
@@ -462,8 +464,10 @@ __global__ void __launch_bounds__(kFinish) qp_alpha_kernel(QpSolveScalars *sc, c
 __global__ void __launch_bounds__(kFinish) qp_omega_kernel(QpSolveScalars *sc, const double *partials, int nblocks) {
   const int d = blockIdx.x;
   if (!sc[d].active || sc[d].restart) return;
-  const float tt = (float)block_sum_partials(partials + (2 * d) * kQpMaxBlocks, nblocks);
-  const float ts = (float)block_sum_partials(partials + (2 * d + 1) * kQpMaxBlocks, nblocks);
+  const double *const src[2] = {partials + (2 * d) * kQpMaxBlocks, partials + (2 * d + 1) * kQpMaxBlocks};
+  double sum[2];
+  block_sum_partials_strided_n(src, nblocks, 1, sum);
+  const float tt = (float)sum[0], ts = (float)sum[1];
   if (threadIdx.x == 0) sc[d].w = tt > 0.f ? ts / tt : 0.f;
 }
 
@@ -497,8 +501,10 @@ __global__ void __launch_bounds__(kFinish) qp_iter_finish_kernel(QpSolveScalars 
                                                                   int nblocks) {
   const int d = blockIdx.x;
   if (!sc[d].active || sc[d].restart) return;
-  const float rr = (float)block_sum_partials(partials + (2 * d) * kQpMaxBlocks, nblocks);
-  const float r0r = (float)block_sum_partials(partials + (2 * d + 1) * kQpMaxBlocks, nblocks);
+  const double *const src[2] = {partials + (2 * d) * kQpMaxBlocks, partials + (2 * d + 1) * kQpMaxBlocks};
+  double sum[2];
+  block_sum_partials_strided_n(src, nblocks, 1, sum);
+  const float rr = (float)sum[0], r0r = (float)sum[1];
   if (threadIdx.x) return;
   QpSolveScalars s = sc[d];
   s.iter += 1;

@@ -47,4 +47,45 @@ __device__ __forceinline__ double block_sum_partials_strided(const double *p, in
   return block_sum(s);
 }
 
+// block_sum of N values with one pair of barriers; component j goes through
+// the same shuffle tree and the same wave order as block_sum, so the results
+// are those of N separate calls.
+template <int N>
+__device__ __forceinline__ void block_sum_n(double (&v)[N]) {
+  __shared__ double red[N][16];
+  __shared__ double total[N];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < N; ++j) v[j] = wave_sum(v[j]);
+  if (lane == 0)
+#pragma unroll
+    for (int j = 0; j < N; ++j) red[j][wave] = v[j];
+  __syncthreads();
+  if (threadIdx.x < N) {
+    double s = 0;
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 0; i < nw; ++i) s += red[threadIdx.x][i];
+    total[threadIdx.x] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < N; ++j) v[j] = total[j];
+  __syncthreads();
+}
+
+// N sums of per-block partials at once: sum j adds p[j][i * stride] over
+// i < n in the order of block_sum_partials_strided.  The loads of all N
+// streams are in flight together.
+template <int N>
+__device__ __forceinline__ void block_sum_partials_strided_n(const double *const (&p)[N], int n, int stride,
+                                                            double (&out)[N]) {
+#pragma unroll
+  for (int j = 0; j < N; ++j) out[j] = 0;
+#pragma unroll 4
+  for (int i = threadIdx.x; i < n; i += blockDim.x)
+#pragma unroll
+    for (int j = 0; j < N; ++j) out[j] += p[j][(size_t)i * stride];
+  block_sum_n(out);
+}
+
 }  // namespace ep

@@ -330,8 +330,10 @@ __global__ void __launch_bounds__(kBlock) wl_group_kernel(WlClasses cls, LanePin
 __global__ void __launch_bounds__(kFinish) reduce_wl_kernel(int nslots, double *partials, DevScalars *sc,
                                                             int lambda_update) {
   if (sc->done) return;
-  double hp = block_sum_partials_strided(partials + kPartialWL, nslots, 2);
-  double wv = block_sum_partials_strided(partials + kPartialWL + 1, nslots, 2);
+  const double *const src[2] = {partials + kPartialWL, partials + kPartialWL + 1};
+  double sum[2];
+  block_sum_partials_strided_n(src, nslots, 2, sum);
+  const double hp = sum[0], wv = sum[1];
   if (threadIdx.x == 0) {
     sc->hpwl = hp;
     sc->wl_value = wv;
