@@ -47,7 +47,11 @@ constexpr double kEps = 1.0e-15;  // global.h:39
 constexpr int kBlock = 256;
 constexpr int kFinish = 1024;  // single-block kernels that sum per-block partials
 constexpr int kNetLaneMax = 32;  // nets above this get a wave each; the rest a power-of-two lane group each
-constexpr int kBigObjectBins = 16;      // objects that may cover more bins get a block each
+// Objects that may cover more bins get a block each; smaller ones go through
+// the tile-sorted scatter and gather, one lane per object (a lane's serial
+// loop over up to 64 bins costs less than a block per object, and it sums
+// in the reference's order).
+constexpr int kBigObjectBins = 64;
 constexpr int kXcds = 8;  // MI355X: workgroups go to the 8 XCDs round-robin
 constexpr int kEvPerEval = 9;  // phase events recorded per evaluation
 // timed phases of an evaluation: scatter, density finalise, wiring kernels,

@@ -31,6 +31,7 @@ import pytest
 from eplace_amd import EPlacer2D, synthetic
 from eplace_amd.flow import place
 from oracle.oracle import Oracle
+from placement_checks import macro_overlap_extent
 
 pytestmark = pytest.mark.gpu
 
@@ -71,21 +72,6 @@ def test_200k_cells_512_bins_against_fp64_oracle():
             assert e < 1e-5
     finally:
         gp.close()
-
-
-def macro_overlap_extent(d, xy):
-    """Largest min(x overlap, y overlap) over macro pairs (0: none)."""
-    m = d.node_is_macro().astype(bool)
-    w, h = d.node_w[m].astype(np.float64), d.node_h[m].astype(np.float64)
-    l, b = xy[m, 0] - w / 2, xy[m, 1] - h / 2
-    worst = 0.0
-    for i in range(len(l)):
-        ox = np.minimum(l[i] + w[i], l + w) - np.maximum(l[i], l)
-        oy = np.minimum(b[i] + h[i], b + h) - np.maximum(b[i], b)
-        ox[i] = 0.0
-        e = np.where((ox > 0) & (oy > 0), np.minimum(ox, oy), 0.0)
-        worst = max(worst, float(e.max()))
-    return worst
 
 
 def run_mixed(cells, nets, macros, bins, util, seed, legal, **kw):
