@@ -354,9 +354,11 @@ static int g_small_blocks(const Ctx &c) {
   const int wpb = kBlock / 64;
   const int groups = c.wl_groups;
   int b = (groups + wpb - 1) / wpb;
-  // four blocks per CU leave room for the transform GEMM blocks running
-  // beside this kernel on the other stream
-  b = b < 1 ? 1 : (b > 1024 ? 1024 : b);
+  // three blocks per CU leave room for the transform GEMM blocks running
+  // beside this kernel on the other stream (measured at 1M cells: with four
+  // the GEMMs are slowed more than the wiring kernel gains, with two the
+  // field gather and node sums lose more than the GEMMs gain)
+  b = b < 1 ? 1 : (b > 768 ? 768 : b);
   return (b + kXcds - 1) / kXcds * kXcds;
 }
 static int g_large_blocks(const Ctx &c) {
