@@ -5,16 +5,23 @@ Workload: a seeded synthetic 1M-cell design in the std-cell recipe (widths
 lognormal median 3 sites, unit height, 1K boundary IO pins, 1.05M nets with
 degree 2:55% 3:20% 4-10:20% 11-200 power-law, index-local pins, utilisation
 0.7, target density 1.0; the reference's bin rule gives 1024x1024 bins).
-A step is one accepted Nesterov iteration of global placement (the plain
-backtracking method of nesterov.hpp:310-369 by default) with every state
-array resident in HBM.  `value` = K timed steps / wall time.  The same run
-also reports the wall time from the first gradient evaluation to overflow
-<= 0.10 for plain backtracking and for the learned controller
-(eplace_amd/controller_params.npy), and the CPU baseline: the oracle's restatement of the reference path
-(reference float arithmetic, one thread) on a bounded sample of the same
-design.
+The transforms run split-FP16 (three fp16 MFMA products per FP32 product,
+FP32 accumulation); everything else is FP32 with FP64 scalars.
 
-    python bench.py [--steps K] [--warmup W] [--method bktrk|vanilla]
+A step is one accepted Nesterov iteration of global placement with the
+learned step / lambda controller (backtracking with the controller's first
+trial and the plain method as fallback, nesterov.hpp:310-369), every state
+array resident in HBM.  `value` = K timed steps from the initial placement /
+wall time.  The metric's own quantity, wall time from the first gradient
+evaluation to overflow <= 0.10 for the learned controller and for plain
+backtracking on the same design, is reported beside it
+(`time_to_overflow_s`, `speedup_learned_vs_plain`, HPWL gaps), together with
+the CPU baseline: the reference's own GP (oracle/_ref/ref_harness, built
+from the reference's sources; one thread, as the reference's GP has no
+threading) on a bounded sample of the same design, and the oracle's
+restatement of it beside.
+
+    python bench.py [--steps K] [--warmup W] [--poisson-mode 0|1|2|3]
 """
 import argparse
 import ctypes
@@ -141,21 +148,36 @@ def per_launch(v, launches):
     return None if v is None else v / launches
 
 
-def roofline(name, seconds, info, poisson_mode=0):
+POISSON_KERNEL = {0: "pgemm_kernel<0> x4 (FP32 MFMA)", 1: "pgemm_kernel<1> x4 (split-BF16, 3 products)",
+                  2: "pgemm_kernel<2> x4 (split-FP16, 3 products)", 3: "pgemm_kernel<3> x4 (split-BF16x3, 6 products)"}
+POISSON_PRODUCTS = {0: 1, 1: 3, 2: 3, 3: 6}
+
+
+def algorithmic_poisson_flops(info):
+    """The transforms' algorithmic work: six dense N x N x N products per
+    evaluation (rho^ forward in x and y, then Ex and Ey inverse in x and y;
+    phi is not consumed), 2 N^3 flop each, on the grid's bins."""
+    nx, ny = info["bins_x"], info["bins_y"]
+    return 6.0 * 2.0 * nx * ny * (nx + ny) / 2.0
+
+
+def roofline(name, seconds, info, poisson_mode=2):
     """Achieved rate of one evaluation phase against its roofline: HBM bytes
     for the streaming phases, MFMA flops for the Poisson transforms."""
     O, N, P = info["num_objects"], info["num_nodes"], info["num_pins"]
     nb = info["bins_x"] * info["bins_y"]
     if name == "poisson":
-        flops = poisson_flops(info)
-        if poisson_mode == 1:  # three bf16 products per FP32 product
-            flops, peak, kern = 3 * flops, BF16_MFMA_PEAK_TFLOPS, "pgemm_kernel<1> x4 (split-BF16)"
-        else:
-            peak, kern = FP32_MFMA_PEAK_TFLOPS, "pgemm_kernel<0> x4 (FP32)"
+        flops = algorithmic_poisson_flops(info)
+        peak = FP32_MFMA_PEAK_TFLOPS if poisson_mode == 0 else BF16_MFMA_PEAK_TFLOPS
         ach = flops / seconds / 1e12
+        issued = poisson_flops(info) * POISSON_PRODUCTS[poisson_mode]
         return {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
-                "traffic": per_launch(phase_traffic(name, info), 4), "kernel": kern, "launches": 4,
-                "work_per_launch": flops / 4, "seconds_per_launch": seconds / 4}
+                "traffic": per_launch(phase_traffic(name, info), 4), "kernel": POISSON_KERNEL[poisson_mode],
+                "launches": 4, "work_per_launch": flops / 4, "seconds_per_launch": seconds / 4,
+                "work": "6 transform products of 2 N^3 flop per evaluation",
+                "issued_matrix_tflops": issued / seconds / 1e12,
+                "issued_note": "parity-folded GEMMs (half the algorithmic multiply-adds) times the MFMA products "
+                               "issued per FP32 product"}
     byts = PHASE_BYTES[name](O, N, P, nb)
     ach = byts / seconds / 1e9
     return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
@@ -163,7 +185,9 @@ def roofline(name, seconds, info, poisson_mode=0):
             "work_per_launch": byts, "seconds_per_launch": seconds}
 
 
-def cpu_baseline(design, method, iters, init_positions):
+def port_baseline(design, method, iters, init_positions):
+    """The oracle's C++ restatement of the reference GP in reference float
+    arithmetic, one thread, `iters` Nesterov iterations after initialisation."""
     from oracle.oracle import Oracle
     o = Oracle(design, 1.0, mode="ref")
     o.set("positions", init_positions.reshape(-1).astype(np.float64))
@@ -181,26 +205,64 @@ def cpu_baseline(design, method, iters, init_positions):
             "seconds": dt}
 
 
-def qp_reference_baseline(design, outer_iterations):
+def reference_gp_start(aux, out_dir, iters):
+    """Starts the reference's own GP (EPlacer_2D + EplaceNesterovOpt -bktrk,
+    built from its sources into oracle/_ref/ref_harness) on the same Bookshelf
+    design in a background process: initialisation, then `iters` timed
+    iterations on one thread.  None when the harness is not built."""
+    import subprocess
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if not os.path.exists(exe):
+        return None
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    cmd = [exe, "time", "-aux", aux, "-noQP", "-targetDensity", "1.0", "-bktrk", "-iters", str(iters),
+           "-out", out_dir]
+    return subprocess.Popen(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True,
+                            cwd=os.path.dirname(aux), env=env)
+
+
+def reference_gp_finish(proc, out_dir, cells, timeout=900):
+    if proc is None:
+        return None
+    try:
+        _, err = proc.communicate(timeout=timeout)
+    except Exception:
+        proc.kill()
+        proc.communicate()
+        return None
+    if proc.returncode != 0:
+        print(f"[bench] reference GP baseline failed: {err[-500:]}", file=sys.stderr, flush=True)
+        return None
+    sc = {}
+    for line in open(os.path.join(out_dir, "scalars.txt")):
+        k, v = line.split()
+        sc[k] = float(v)
+    it, dt = int(sc["iterations"]), sc["iter_time"]
+    return {"value": it / dt, "unit": "iters/s", "cores": 1, "kind": "reference",
+            "sample": f"{it} -bktrk Nesterov iterations (EplaceNesterovOpt::opt_step_bktrk over "
+                      f"EPlacer_2D::totalGradientUpdate, built from the reference's sources) of the same "
+                      f"{cells}-cell design from the same initial placement, after the reference's own "
+                      f"initialisation ({sc['init_time']:.1f}s, not timed); 1 thread on {cpu_model()}",
+            "seconds": dt, "seconds_per_gradient": sc["grad_time"], "init_seconds": sc["init_time"]}
+
+
+def qp_reference_baseline(aux, outer_iterations):
     """The reference's own QPPlacer (oracle/_ref/ref_harness, built from the
     reference's sources with its vendored Eigen, 8 OpenMP threads as
-    qplace.cpp:50 asks) on the same design for `outer_iterations` outer
-    iterations from the core centre; its per-iteration wall time is the
+    qplace.cpp:50 asks) on the Bookshelf design `aux` for `outer_iterations`
+    outer iterations from the core centre; its per-iteration wall time is the
     baseline.  None when the harness is not built."""
     import subprocess
     import tempfile
-    from eplace_amd.design import write_bookshelf
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
     if not os.path.exists(exe):
         return None
     tmp = tempfile.mkdtemp(prefix="qpref_")
     try:
-        aux = write_bookshelf(design, os.path.join(tmp, "in"))
-        out = os.path.join(tmp, "out")
         env = dict(os.environ, OMP_NUM_THREADS="8")
         r = subprocess.run([exe, "qp", "-noQP", "-timeOnly", "-aux", aux, "-IPiteCount",
-                            str(outer_iterations - 1), "-out", out], capture_output=True, text=True, cwd=tmp,
-                           env=env, timeout=600)
+                            str(outer_iterations - 1), "-out", tmp], capture_output=True, text=True,
+                           cwd=os.path.dirname(aux), env=env, timeout=600)
         if r.returncode != 0:
             return None
         per = [float(l.split("CPUtime")[1]) for l in r.stdout.splitlines() if "at iteration number" in l]
@@ -217,22 +279,37 @@ def qp_reference_baseline(design, outer_iterations):
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+def fp64_plain_reference(cells, seed):
+    """Final HPWL and iteration count of plain backtracking GP to overflow
+    0.10 computed by the FP64 oracle on this workload (a fixture written by
+    tests/golden/gen_fp64_plain.py), or None."""
+    path = os.path.join(ROOT, "tests", "golden", "fp64_plain_gp.json")
+    try:
+        with open(path) as f:
+            runs = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for r in runs.get("runs", []):
+        if r.get("cells") == cells and r.get("seed") == seed and r.get("bins") in (None, 0):
+            return r
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cells", type=int, default=1_000_000)
-    ap.add_argument("--method", default="bktrk", choices=["bktrk", "vanilla"])
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-converge", action="store_true")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-learned", action="store_true", help="skip the learned-controller convergence run")
-    ap.add_argument("--cpu-iters", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--cpu-iters", type=int, default=10, help="iterations of the oracle restatement (port)")
+    ap.add_argument("--cpu-ref-iters", type=int, default=5, help="iterations of the reference's own GP")
     ap.add_argument("--no-qp", action="store_true", help="skip the quadratic-placement run")
     ap.add_argument("--no-qp-cpu", action="store_true", help="skip the reference QP CPU baseline")
-    ap.add_argument("--poisson-mode", type=int, default=1, choices=[0, 1],
-                    help="transform arithmetic: 0 FP32 MFMA, 1 split-BF16 MFMA")
+    ap.add_argument("--poisson-mode", type=int, default=2, choices=[0, 1, 2, 3],
+                    help="transform arithmetic: 0 FP32 MFMA, 1 split-BF16, 2 split-FP16, 3 3-plane split-BF16")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (scales HBM counters), then exit")
     args = ap.parse_args()
@@ -252,7 +329,10 @@ def main():
         print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 5}))
         return
 
+    import shutil
+    import tempfile
     from eplace_amd import EPlacer2D, EplaceNesterovOpt, synthetic
+    from eplace_amd.controller import Controller
 
     cells = args.cells
     nets = int(round(cells * 1.05))
@@ -261,6 +341,18 @@ def main():
     t_gen = time.perf_counter() - t
     print(f"[bench] design {cells} cells, {nets} nets, {d.num_pins} pins (generated in {t_gen:.1f}s)",
           file=sys.stderr, flush=True)
+
+    # the reference's own GP on the same design, in the background (host only)
+    tmp = tempfile.mkdtemp(prefix="bench_ref_")
+    ref_proc, aux = None, None
+    if not args.no_cpu:
+        from eplace_amd.design import write_bookshelf
+        t = time.perf_counter()
+        aux = write_bookshelf(d, os.path.join(tmp, "in"))
+        ref_proc = reference_gp_start(aux, os.path.join(tmp, "ref_gp"), args.cpu_ref_iters)
+        print(f"[bench] Bookshelf written in {time.perf_counter() - t:.1f}s; reference GP baseline "
+              f"{'started' if ref_proc else 'not built'}", file=sys.stderr, flush=True)
+
     t = time.perf_counter()
     gp = EPlacer2D(d, 1.0, poisson_mode=args.poisson_mode)
     info = gp.info()
@@ -268,26 +360,30 @@ def main():
     print(f"[bench] engine ready in {t_create:.1f}s: {info['num_objects']} objects "
           f"({info['num_fillers']} fillers), {info['bins_x']}x{info['bins_y']} bins", file=sys.stderr, flush=True)
     init_pos = gp.read("positions").copy()
+    ctrl = Controller.load()
 
     def restart():
         gp.write("positions", init_pos)
         gp.initialization()
 
-    # warmup: W iterations from the initial placement (discarded)
-    gp.initialization()
-    EplaceNesterovOpt(gp, method=args.method, max_iterations=max(args.warmup - 1, 0)).opt()
+    def learned(**kw):
+        return EplaceNesterovOpt(gp, method="learned", controller=ctrl, log=False, **kw)
+
+    # warmup: W learned iterations from the initial placement (discarded)
+    restart()
+    learned(max_iterations=max(args.warmup - 1, 0)).opt()
     print("[bench] warmup done", file=sys.stderr, flush=True)
 
-    # timed: exactly K iterations from the initial placement (evaluations
-    # replayed from their captured graphs)
+    # timed: exactly K learned iterations from the initial placement
+    # (evaluations replayed from their captured graphs)
     restart()
     device_sync()
     t0 = time.perf_counter()
-    r = EplaceNesterovOpt(gp, method=args.method, max_iterations=args.steps - 1, log=False).opt()
+    r = learned(max_iterations=args.steps - 1).opt()
     device_sync()
     dt = time.perf_counter() - t0
     steps = r["iterations"]
-    print(f"[bench] timed {steps} iterations in {dt:.3f}s, {r['evaluations']} evaluations, "
+    print(f"[bench] timed {steps} learned iterations in {dt:.4f}s, {r['evaluations']} evaluations, "
           f"{r['backtracks']} backtracks", file=sys.stderr, flush=True)
     # phase times: the same K iterations from the same start once more, every
     # evaluation launched kernel by kernel with HIP events between its phases
@@ -296,7 +392,7 @@ def main():
     restart()
     device_sync()
     t0 = time.perf_counter()
-    r = EplaceNesterovOpt(gp, method=args.method, max_iterations=args.steps - 1, log=False).opt(profile=True)
+    r = learned(max_iterations=args.steps - 1).opt(profile=True)
     device_sync()
     dt_events = time.perf_counter() - t0
     ph = r["phase_seconds"]
@@ -308,50 +404,48 @@ def main():
     roof_all = {k: roofline(k, ph[k], info, args.poisson_mode) for k in ph}
 
     conv = None
+    headline = {}
     if not args.no_converge:
-        restart_t0 = time.perf_counter()
-        gp.write("positions", init_pos)
-        device_sync()
-        t0 = time.perf_counter()
-        gp.initialization()
-        rc = EplaceNesterovOpt(gp, method=args.method, target_overflow=0.10, log=False).opt()
-        device_sync()
-        tc = time.perf_counter() - t0
-        conv = {"seconds_to_overflow_0.10": tc, "iterations": rc["iterations"], "evaluations": rc["evaluations"],
-                "backtracks": rc["backtracks"], "final_overflow": rc["final_overflow"],
-                "final_hpwl": rc["final_hpwl"], "iters_per_s": rc["iterations"] / tc}
-        print(f"[bench] converged: {rc['iterations']} iterations in {tc:.3f}s, overflow {rc['final_overflow']:.4f}, "
-              f"HPWL {rc['final_hpwl']:.6g}", file=sys.stderr, flush=True)
-        del restart_t0
-        from eplace_amd.legalize import check_legal, legalize_abacus
-        t0 = time.perf_counter()
-        lxy, lst = legalize_abacus(d, gp.read("positions")[: d.num_nodes], mode=0)
-        chk = check_legal(d, lxy)
-        conv["legalization"] = {"seconds": time.perf_counter() - t0, "legal": chk["legal"],
-                                "mean_displacement": lst["total_displacement"] / max(lst["cells"], 1),
-                                "max_displacement": lst["max_displacement"], "check": chk}
-        print(f"[bench] Abacus: {lst['cells']} cells in {conv['legalization']['seconds']:.2f}s, "
-              f"{'legal' if chk['legal'] else 'NOT legal'}, mean displacement "
-              f"{conv['legalization']['mean_displacement']:.3f}", file=sys.stderr, flush=True)
-        if args.method == "bktrk" and not args.no_learned:
-            from eplace_amd.controller import Controller
-            ctrl = Controller.load()
-            gp.write("positions", init_pos)
+        runs = {}
+        for method in ("bktrk", "learned"):
+            restart()
             device_sync()
             t0 = time.perf_counter()
-            gp.initialization()
-            rl = EplaceNesterovOpt(gp, method="learned", controller=ctrl, target_overflow=0.10,
-                                   log=False).opt()
+            if method == "learned":
+                rc = learned(target_overflow=0.10).opt()
+            else:
+                rc = EplaceNesterovOpt(gp, method="bktrk", target_overflow=0.10, log=False).opt()
             device_sync()
-            tl = time.perf_counter() - t0
-            conv["learned"] = {"seconds_to_overflow_0.10": tl, "iterations": rl["iterations"],
-                               "evaluations": rl["evaluations"], "backtracks": rl["backtracks"],
-                               "controller_fallbacks": rl["controller_fallbacks"],
-                               "final_overflow": rl["final_overflow"], "final_hpwl": rl["final_hpwl"],
-                               "speedup_vs_plain": tc / tl, "hpwl_gap_vs_plain": rl["final_hpwl"] / rc["final_hpwl"] - 1}
-            print(f"[bench] learned: {rl['iterations']} iterations in {tl:.3f}s ({tc / tl:.2f}x), "
-                  f"overflow {rl['final_overflow']:.4f}, HPWL {rl['final_hpwl']:.6g} "
-                  f"({100 * (rl['final_hpwl'] / rc['final_hpwl'] - 1):+.3f}%)", file=sys.stderr, flush=True)
+            tc = time.perf_counter() - t0
+            runs[method] = dict(seconds=tc, iterations=rc["iterations"], evaluations=rc["evaluations"],
+                                backtracks=rc["backtracks"], final_overflow=rc["final_overflow"],
+                                final_hpwl=rc["final_hpwl"], iters_per_s=rc["iterations"] / tc,
+                                controller_fallbacks=rc.get("controller_fallbacks", 0))
+            print(f"[bench] {method}: {rc['iterations']} iterations ({rc['evaluations']} evaluations) in "
+                  f"{tc:.3f}s, overflow {rc['final_overflow']:.4f}, HPWL {rc['final_hpwl']:.6g}",
+                  file=sys.stderr, flush=True)
+            if method == "bktrk":
+                from eplace_amd.legalize import check_legal, legalize_abacus
+                t0 = time.perf_counter()
+                lxy, lst = legalize_abacus(d, gp.read("positions")[: d.num_nodes], mode=0)
+                chk = check_legal(d, lxy)
+                runs[method]["legalization"] = {
+                    "seconds": time.perf_counter() - t0, "legal": chk["legal"],
+                    "mean_displacement": lst["total_displacement"] / max(lst["cells"], 1),
+                    "max_displacement": lst["max_displacement"], "check": chk}
+                print(f"[bench] Abacus: {lst['cells']} cells, {'legal' if chk['legal'] else 'NOT legal'}",
+                      file=sys.stderr, flush=True)
+        pl, le = runs["bktrk"], runs["learned"]
+        conv = {"plain_bktrk": pl, "learned": le}
+        headline = {"time_to_overflow_0.10_s": {"learned": le["seconds"], "plain": pl["seconds"]},
+                    "speedup_learned_vs_plain": pl["seconds"] / le["seconds"],
+                    "hpwl_gap_learned_vs_plain": le["final_hpwl"] / pl["final_hpwl"] - 1,
+                    "evaluations_to_overflow_0.10": {"learned": le["evaluations"], "plain": pl["evaluations"]}}
+        f64 = fp64_plain_reference(cells, args.seed)
+        if f64 is not None:
+            headline["hpwl_gap_learned_vs_fp64_plain"] = le["final_hpwl"] / f64["final_hpwl"] - 1
+            headline["hpwl_gap_plain_vs_fp64_plain"] = pl["final_hpwl"] / f64["final_hpwl"] - 1
+            headline["fp64_plain_reference"] = {k: f64[k] for k in ("final_hpwl", "iterations", "final_overflow")}
 
     qp = None
     if not args.no_qp:
@@ -366,17 +460,9 @@ def main():
                           "-IPiteCount 20, BiCGSTAB 100 iterations)",
               "seconds": tq, "iterations": sq["iterations"], "solver_iterations": sq["solver_iterations"],
               "seconds_per_outer_iteration": tq / sq["iterations"], "initial_hpwl": sq["initial_hpwl"],
-              "final_hpwl": sq["final_hpwl"], "error_x": sq["error_x"], "error_y": sq["error_y"],
-              "hpwl_log": sq["log"][:, 0].tolist()}
+              "final_hpwl": sq["final_hpwl"], "error_x": sq["error_x"], "error_y": sq["error_y"]}
         print(f"[bench] QP: {sq['iterations']} outer iterations ({sq['solver_iterations']} BiCGSTAB) in {tq:.3f}s, "
               f"HPWL {sq['initial_hpwl']:.6g} -> {sq['final_hpwl']:.6g}", file=sys.stderr, flush=True)
-        if not args.no_qp_cpu and not args.no_cpu:
-            base = qp_reference_baseline(dq, 1)
-            if base is not None:
-                qp["cpu_baseline"] = base
-                qp["speedup_per_outer_iteration"] = base["value"] / qp["seconds_per_outer_iteration"]
-                print(f"[bench] QP reference: {base['value']:.2f}s per outer iteration "
-                      f"({qp['speedup_per_outer_iteration']:.0f}x)", file=sys.stderr, flush=True)
         del dq
 
     copy_gbs = copy_peak_gbs()
@@ -385,12 +471,34 @@ def main():
         if v["bound"] == "hbm":
             v["copy_peak"] = copy_gbs
             v["frac_of_copy"] = v["achieved"] / copy_gbs
+    gp.close()
 
-    cpu = None
+    cpu, port = None, None
     if not args.no_cpu:
-        cpu = cpu_baseline(d, args.method, args.cpu_iters, init_pos)
-        print(f"[bench] cpu baseline {cpu['value']:.4f} iters/s over {cpu['seconds']:.1f}s", file=sys.stderr,
+        if qp is not None and not args.no_qp_cpu and aux is not None:
+            base = qp_reference_baseline(aux, 1)
+            if base is not None:
+                qp["cpu_baseline"] = base
+                qp["speedup_per_outer_iteration"] = base["value"] / qp["seconds_per_outer_iteration"]
+                print(f"[bench] QP reference: {base['value']:.2f}s per outer iteration "
+                      f"({qp['speedup_per_outer_iteration']:.0f}x)", file=sys.stderr, flush=True)
+        port = port_baseline(d, "bktrk", args.cpu_iters, init_pos)
+        print(f"[bench] port baseline {port['value']:.4f} iters/s over {port['seconds']:.1f}s", file=sys.stderr,
               flush=True)
+        cpu = reference_gp_finish(ref_proc, os.path.join(tmp, "ref_gp"), cells)
+        if cpu is not None:
+            print(f"[bench] reference GP baseline {cpu['value']:.4f} iters/s over {cpu['seconds']:.1f}s",
+                  file=sys.stderr, flush=True)
+        else:
+            cpu = port
+        if conv is not None:
+            n_plain = conv["plain_bktrk"]["iterations"]
+            est = n_plain / cpu["value"]
+            headline["cpu_time_to_overflow_0.10_s_estimate"] = est
+            headline["cpu_estimate_note"] = (f"{cpu['kind']} CPU path: {n_plain} plain iterations (the GPU's plain "
+                                             f"count) at the measured {cpu['value']:.4f} iters/s")
+            headline["speedup_learned_vs_cpu"] = est / conv["learned"]["seconds"]
+    shutil.rmtree(tmp, ignore_errors=True)
 
     out = {
         "metric": "GP time to overflow<=0.10 @1M cells, learned vs plain (HPWL +-0.5%); iters/s",
@@ -404,13 +512,20 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "fp32+split-fp16" if args.poisson_mode == 2 else
+                 {0: "fp32", 1: "fp32+split-bf16", 3: "fp32+split-bf16x3"}[args.poisson_mode],
+        "dtype_note": "positions, maps, fields and gradients FP32; density sums int64 fixed point; transforms "
+                      + {0: "FP32 MFMA", 1: "split-BF16 MFMA (3 products)", 2: "split-FP16 MFMA (3 products, "
+                         "power-of-two operand scales)", 3: "split-BF16 MFMA (3 planes, 6 products)"}[args.poisson_mode]
+                      + " accumulated in FP32; HPWL, energy, step and lambda scalars FP64",
         "data": "synthetic",
         "config": {"workload": f"synthetic std-cell design, {cells} cells, {nets} nets, {d.num_pins} pins, "
                                f"{info['num_fillers']} fillers, {info['bins_x']}x{info['bins_y']} bins, "
                                f"util 0.7, target density 1.0, seed {args.seed}",
-                   "method": f"plain {args.method} Nesterov (device-resident)", "cells": cells,
-                   "bins": [info["bins_x"], info["bins_y"]]},
+                   "method": "learned step/lambda controller over backtracking Nesterov (device-resident); "
+                             "timed K iterations from the initial placement",
+                   "cells": cells, "bins": [info["bins_x"], info["bins_y"]]},
+        **headline,
         "roofline": roof,
         "roofline_phases": {k: {"bound": v["bound"], "achieved": v["achieved"], "unit": v["unit"],
                                 "frac": v["frac"], "frac_of_copy": v.get("frac_of_copy"), "traffic": v["traffic"]}
@@ -419,6 +534,7 @@ def main():
         "converge": conv,
         "qp": qp,
         "cpu_baseline": cpu,
+        "cpu_baseline_port": port,
     }
     print(json.dumps(out))
 

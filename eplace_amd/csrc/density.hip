@@ -262,7 +262,7 @@ __global__ void __launch_bounds__(kBlock) finalize_density_kernel(
     float *__restrict__ rho, double *partials, const DevScalars *sc) {
   if (sc->done) return;
   int k = blockIdx.x * blockDim.x + threadIdx.x;
-  double ovf = 0, rsum = 0;
+  double ovf = 0, rsum = 0, rmax = 0;
   if (k < nb) {
     float node = (float)((double)(long long)node_q[k] * inv_scale);
     float fill = (float)((double)(long long)filler_q[k] * inv_scale);
@@ -272,14 +272,17 @@ __global__ void __launch_bounds__(kBlock) finalize_density_kernel(
     float r = e * g.inv_bin_area;
     rho[(size_t)i * ldp + j] = r;
     rsum = r;
+    rmax = fabs((double)r);
     float d = (node + t + b) * g.inv_bin_area - g.td;
     ovf = (double)(fmaxf(0.f, d) * g.bin_area);
   }
   double s = block_sum(ovf);
   double rs = block_sum(rsum);
+  double rm = block_max(rmax);
   if (threadIdx.x == 0) {
     partials[kPartialTau + blockIdx.x] = s;
     partials[kPartialRho + blockIdx.x] = rs;
+    partials[kPartialRhoMax + blockIdx.x] = rm;
   }
 }
 
@@ -294,10 +297,14 @@ __global__ void __launch_bounds__(kFinish) finish_density_kernel(int nblocks, in
   double sum[2];
   block_sum_partials_strided_n(src, nblocks, 1, sum);
   const double tot = sum[0], rtot = sum[1];
+  double rmax = 0;
+  for (int i = threadIdx.x; i < nblocks; i += blockDim.x) rmax = fmax(rmax, partials[kPartialRhoMax + i]);
+  rmax = block_max(rmax);
   if (threadIdx.x == 0) {
     double tau = tot / node_area_scaled;
     sc->overflow = tau;
     sc->rho_mean = rtot / (double)nb;
+    sc->rho_max = rmax;
     float cx = 0.125f / g.bsx, cy = 0.125f / g.bsy;
     float tf = (float)tau;
     if (tf > 1.0) {

@@ -125,8 +125,9 @@ int validate(Ctx &c, const ep_design *d, const ep_params *p) {
   // partial-sum slots bound the grid: nx*ny/256 and objects/256 <= kMaxPartials
   if (p->bins_x < 0 || p->bins_y < 0 || p->bins_x > 2048 || p->bins_y > 2048)
     return c.err = "bins out of range [0, 2048]", EP_ERR_ARG;
-  if (p->poisson_mode != 0 && p->poisson_mode != 1)
-    return c.err = "poisson_mode must be 0 (FP32 MFMA) or 1 (split-BF16 MFMA)", EP_ERR_ARG;
+  if (p->poisson_mode < 0 || p->poisson_mode > 3)
+    return c.err = "poisson_mode must be 0 (FP32), 1 (split-BF16), 2 (split-FP16) or 3 (3-plane split-BF16)",
+           EP_ERR_ARG;
   if ((p->bins_x > 0 && p->bins_x % 2) || (p->bins_y > 0 && p->bins_y % 2))
     return c.err = "bins per side must be even", EP_ERR_ARG;
   if (p->wirelength_model < 0 || p->wirelength_model > 1 || p->exp_mode < 0 || p->exp_mode > 1)
@@ -648,9 +649,11 @@ int update_scalars(Ctx &c, F f) {
   return EP_OK;
 }
 
+// HPWL (and the optional lambda update) at the current positions; the
+// wiring kernels run without gradients, so the stored WA value is kept.
 int hpwl_pass(Ctx &c, int lambda_update) {
   launch_wirelength(c, false);
-  launch_reduce_wirelength(c, lambda_update);
+  launch_reduce_wirelength(c, lambda_update, nullptr, false);
   return sync(c);
 }
 
@@ -705,6 +708,15 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
   if (o.method == 2 && !ctrl) return c.err = "method 2 needs a controller", EP_ERR_ARG;
   if (ctrl && ctrl_device(ctrl) != c.device) return c.err = "controller is on another device", EP_ERR_ARG;
   double *dlog = nullptr;
+  // frees the device log and ends phase profiling on every exit path
+  struct RunScope {
+    Ctx &c;
+    double *&dlog;
+    ~RunScope() {
+      if (dlog) (void)hipFree(dlog);
+      c.profiling = false;
+    }
+  } scope{c, dlog};
   int cap = o.log_capacity;
   if (cap > 0) {
     EP_HIP(hipMalloc(&dlog, (size_t)cap * EP_LOG_COLS * sizeof(double)));
@@ -853,10 +865,7 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
     for (int k = 0; k < kPhases; ++k) o.phase_seconds[k] = acc[k];
     o.phase_seconds[kPhases] = (double)n;
   }
-  if (cap > 0) {
-    EP_HIP(hipMemcpy(o.log, dlog, (size_t)cap * EP_LOG_COLS * sizeof(double), hipMemcpyDeviceToHost));
-    (void)hipFree(dlog);
-  }
+  if (cap > 0) EP_HIP(hipMemcpy(o.log, dlog, (size_t)cap * EP_LOG_COLS * sizeof(double), hipMemcpyDeviceToHost));
   if (out) {
     out->iterations = iters;
     out->evaluations = h.evals;
@@ -1032,10 +1041,8 @@ int ep_get_stats(ep_ctx *x, ep_stats *s) {
 int ep_hpwl(ep_ctx *x, double *out) {
   if (!x || !out) return EP_ERR_ARG;
   Ctx &c = x->c;
-  ep::DevScalars keep;
-  int rc = ep::read_scalars(c, keep);
+  int rc = ep::hpwl_pass(c, 0);
   if (rc) return rc;
-  if ((rc = ep::hpwl_pass(c, 0))) return rc;
   ep::DevScalars h;
   if ((rc = ep::read_scalars(c, h))) return rc;
   *out = h.hpwl;
@@ -1056,10 +1063,17 @@ int ep_poisson(ep_ctx *x, const float *rho, float *phi, float *ex, float *ey) {
     std::memcpy(&buf[(size_t)i * ld], rho + (size_t)i * c.ny, (size_t)c.ny * 4);
   int rc = ep::sync(c);
   if (rc) return rc;
-  double mean = 0;
-  for (size_t k = 0; k < (size_t)c.nx * c.ny; ++k) mean += rho[k];
+  double mean = 0, mx = 0;
+  for (size_t k = 0; k < (size_t)c.nx * c.ny; ++k) {
+    mean += rho[k];
+    mx = std::max(mx, std::fabs((double)rho[k]));
+  }
   mean /= (double)c.nx * c.ny;
-  if ((rc = ep::update_scalars(c, [&](ep::DevScalars &h) { h.rho_mean = mean; }))) return rc;
+  if ((rc = ep::update_scalars(c, [&](ep::DevScalars &h) {
+         h.rho_mean = mean;
+         h.rho_max = mx;
+       })))
+    return rc;
   EP_HIP(hipMemcpy(c.rho, buf.data(), buf.size() * 4, hipMemcpyHostToDevice));
   ep::launch_poisson(c, phi != nullptr);
   if ((rc = ep::sync(c))) return rc;

@@ -95,6 +95,7 @@ struct DevScalars {
   double lip_num, lip_den;
   double init_lambda_num, init_lambda_den;
   double rho_mean;     // mean charge density; removed before the transform (DC is discarded)
+  double rho_max;      // largest charge density (sets the operand scale of the FP16 transforms)
   int iter, done, accepted, stage;
   int evals, backtracks, max_iter, bad_numeric;
   float target_overflow, pad0;
@@ -116,7 +117,8 @@ constexpr int kPartialLip = 3 * kMaxPartials;   // num, den pairs
 constexpr int kPartialLambda = 5 * kMaxPartials;  // |gwl|, |gd| pairs
 constexpr int kPartialRho = 7 * kMaxPartials;      // sum of rho per block
 constexpr int kPartialNorm = 8 * kMaxPartials;     // |gwl|^2, |gd|^2 pairs
-constexpr int kPartialTotal = 10 * kMaxPartials;
+constexpr int kPartialRhoMax = 10 * kMaxPartials;  // max of rho per block
+constexpr int kPartialTotal = 11 * kMaxPartials;
 
 struct Poisson;  // poisson.hip
 
@@ -273,7 +275,8 @@ void tile_sort_destroy(Ctx &c);
 // ---- wirelength.hip ----
 // st: the stream to launch on (nullptr: the context's stream)
 void launch_wirelength(Ctx &c, bool gradient, hipStream_t st = nullptr);
-void launch_reduce_wirelength(Ctx &c, int lambda_update, hipStream_t st = nullptr);
+// with_value: the wiring pass computed WA values (its gradient variant)
+void launch_reduce_wirelength(Ctx &c, int lambda_update, hipStream_t st = nullptr, bool with_value = true);
 // ---- poisson.hip ----
 bool poisson_create(Ctx &c, std::string &err);
 void poisson_destroy(Ctx &c);

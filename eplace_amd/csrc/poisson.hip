@@ -34,13 +34,29 @@
 // both operands in a double-buffered LDS, one barrier per k step, with the
 // next slice's global loads in flight during the multiply.
 //
-// Two arithmetic modes: FP32 MFMA (16x16x4), and split-BF16: each FP32
-// value is a_hi + a_lo (two bf16; the constant matrices are stored split),
-// and a b ~ a_hi b_hi + a_hi b_lo + a_lo b_hi on bf16 MFMA (16x16x32),
-// accumulated in FP32.
+// Arithmetic modes (ep_params.poisson_mode):
+//   0  FP32 MFMA (16x16x4);
+//   1  split-BF16: each FP32 value is a_hi + a_lo (two bf16; the constant
+//      matrices are stored split), a b ~ a_hi b_hi + a_hi b_lo + a_lo b_hi on
+//      bf16 MFMA (16x16x32), accumulated in FP32; the operands keep 16
+//      significant bits;
+//   2  split-FP16: the same three products on fp16 MFMA (16x16x32).  fp16
+//      halves keep 11 bits each, so a_hi + a_lo carries 22; its exponent
+//      range is narrow, so every operand is multiplied by a power of two
+//      that puts its largest magnitude near 2^13 (exact in FP32, undone in
+//      the epilogue).  The constant matrices have fixed scales; the data
+//      operand of a stage takes its scale from the largest |value| the
+//      previous stage wrote (per-wave maxima written by its epilogue; for
+//      stage 1 the largest charge density).  Values far below the largest
+//      lose only absolute precision below 2^-27 of it, whether or not fp16
+//      subnormals are flushed;
+//   3  split-BF16, three planes: a = a_0 + a_1 + a_2 (24 bits), six products
+//      (every pair of plane orders summing to at most 2).
 #include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
 
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "ep_internal.h"
@@ -49,15 +65,24 @@ namespace ep {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 
-// A constant transform matrix: FP32, plus its bf16 hi/lo split.
+constexpr int kMaxPlanes = 3;
+// Largest |operand| after scaling in mode 2 (folded data adds two values,
+// so the stored halves stay below 2^14 < 65504).
+constexpr int kF16ScaleExp = 13;
+
+// A constant transform matrix: FP32, its bf16 planes and its scaled fp16
+// halves (value * 2^f16_exp).
 struct Mat {
   float *f = nullptr;
-  __bf16 *hi = nullptr, *lo = nullptr;
+  __bf16 *bp[kMaxPlanes] = {nullptr, nullptr, nullptr};
+  _Float16 *hp[2] = {nullptr, nullptr};
+  int f16_exp = 0;
 };
 
 struct Poisson {
-  int mode = 0;                // 0 FP32, 1 split-BF16
+  int mode = 0;                // poisson_mode
   int nxp = 0, nyp = 0;        // padded map sizes (multiples of 64)
   int hx = 0, hy = 0;          // nx/2, ny/2
   int hxp = 0, hyp = 0;        // halves padded to multiples of 64
@@ -71,14 +96,18 @@ struct Poisson {
   float *T1t = nullptr;   // [2hyp x nxp]
   float *spec = nullptr;  // [2hxp x 2hyp]
   float *PQt = nullptr;   // [2 nyp x 2hxp]: Pt rows 0..nyp, Qt rows nyp..2nyp
+  float *wmax = nullptr;  // mode 2: per-wave max |output| of stages 1-3 [3][kMaxWaveMax]
 };
+
+constexpr int kMaxWaveMax = 1 << 16;
 
 int poisson_ld(const Ctx &c) { return c.poisson ? c.poisson->nyp : c.ny; }
 
 // One launch of up to three GEMMs C = A B^T (blockIdx.z selects).
 struct PGemm {
-  const float *Af[3];          // FP32 A (mode 0)
-  const __bf16 *Ah[3], *Al[3];  // split A (mode 1)
+  const float *Af[3];                 // FP32 A (mode 0)
+  const void *Ap[3][kMaxPlanes];      // A planes (modes 1-3)
+  float a_inv[3];                     // mode 2: 2^-f16_exp of A
   const float *B[3];
   float *C[3];
   float sb[3];         // fold sign of B along k (0: no fold)
@@ -93,6 +122,13 @@ struct PGemm {
   int epi;             // 1: divide by wx2p[row] + wy2p[col], zero the DC term
   const double *sub;   // B := B - *sub before folding (removes the mean of rho)
   const float *wx2p, *wy2p;
+  // mode 2: bound on |B| before folding: the maximum of in_max[0..in_n) or,
+  // if in_max_d is set, *in_max_d; out_max (optional) receives one max |C|
+  // per wave of the launch
+  const float *in_max;
+  int in_n;
+  const double *in_max_d;
+  float *out_max;
 };
 
 constexpr int PBM = 64, PBN = 64, PK = 32;
@@ -102,26 +138,40 @@ constexpr int kGroupThreads = 256;
 // Thread index inside the thread's k-group.
 __device__ __forceinline__ int gtid() { return threadIdx.x & (kGroupThreads - 1); }
 constexpr int kLd32 = PK + 4;  // FP32 LDS row stride (floats), rows 16-byte aligned
-// bf16 LDS tiles: 64-byte rows (32 k) of four 16-byte chunks, chunk c of row
-// r stored at c ^ ((r >> 1 ^ r >> 3) & 3).  The stores (whole rows) and the
-// MFMA operand reads (16 consecutive rows, one chunk) then touch distinct
-// 16-byte bank groups per 8 lanes with 32 banks and per 16 lanes with 64.
+// 16-bit LDS tiles: 64-byte rows (32 k) of four 16-byte chunks, chunk c of
+// row r stored at c ^ ((r >> 1 ^ r >> 3) & 3).  The stores (whole rows) and
+// the MFMA operand reads (16 consecutive rows, one chunk) then touch
+// distinct 16-byte bank groups per 8 lanes with 32 banks and per 16 lanes
+// with 64.
 __device__ __forceinline__ int swz16(int r, int c) { return 8 * (c ^ ((r >> 1 ^ r >> 3) & 3)); }
+
+// Element type, vector type and plane count of each split mode.
+template <int MODE> struct Split;
+template <> struct Split<1> { typedef __bf16 T; typedef bf16x8 V; static constexpr int NP = 2; };
+template <> struct Split<2> { typedef _Float16 T; typedef f16x8 V; static constexpr int NP = 2; };
+template <> struct Split<3> { typedef __bf16 T; typedef bf16x8 V; static constexpr int NP = 3; };
+
+__device__ __forceinline__ floatx4 mfma16(bf16x8 a, bf16x8 b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ floatx4 mfma16(f16x8 a, f16x8 b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
 
 // Per-thread share of one k step: row tid / 4 of the 64-row A and B tiles,
 // eight consecutive k at 8 (tid % 4).
 struct Stage32 {
   float a[8], b[8];
 };
-struct Stage16 {
-  bf16x8 ah, al;
+template <int MODE> struct StageS {
+  typename Split<MODE>::V a[Split<MODE>::NP];
   float b[8];
 };
 struct Smem32 {
   float A[2][PBM][kLd32], B[2][PBN][kLd32];
 };
-struct Smem16 {
-  __bf16 Ah[2][PBM][PK], Al[2][PBM][PK], Bh[2][PBN][PK], Bl[2][PBN][PK];
+template <int MODE> struct SmemS {
+  typename Split<MODE>::T A[2][Split<MODE>::NP][PBM][PK], B[2][Split<MODE>::NP][PBN][PK];
 };
 
 __device__ __forceinline__ void load_b(const PGemm &p, const float *__restrict__ B, float sb, float sub, int col0,
@@ -162,16 +212,19 @@ __device__ __forceinline__ void load_stage(const PGemm &p, int z, const float *B
   load_b(p, B, sb, sub, col0, k0, st.b);
 }
 
+template <int MODE>
 __device__ __forceinline__ void load_stage(const PGemm &p, int z, const float *B, float sb, float sub, int row0,
-                                           int col0, int k0, Stage16 &st) {
+                                           int col0, int k0, StageS<MODE> &st) {
+  typedef typename Split<MODE>::T T;
+  typedef typename Split<MODE>::V V;
   const int r = gtid() >> 2, k = k0 + 8 * (gtid() & 3);
   const size_t off = (size_t)(row0 + r) * p.lda + k;
-  st.ah = *reinterpret_cast<const bf16x8 *>(p.Ah[z] + off);
-  st.al = *reinterpret_cast<const bf16x8 *>(p.Al[z] + off);
+#pragma unroll
+  for (int q = 0; q < Split<MODE>::NP; ++q) st.a[q] = *reinterpret_cast<const V *>(static_cast<const T *>(p.Ap[z][q]) + off);
   load_b(p, B, sb, sub, col0, k0, st.b);
 }
 
-__device__ __forceinline__ void store_stage(Smem32 &s, int buf, const Stage32 &st) {
+__device__ __forceinline__ void store_stage(Smem32 &s, int buf, const Stage32 &st, float) {
   const int r = gtid() >> 2, k = 8 * (gtid() & 3);
   *reinterpret_cast<float4 *>(&s.A[buf][r][k]) = make_float4(st.a[0], st.a[1], st.a[2], st.a[3]);
   *reinterpret_cast<float4 *>(&s.A[buf][r][k + 4]) = make_float4(st.a[4], st.a[5], st.a[6], st.a[7]);
@@ -179,19 +232,29 @@ __device__ __forceinline__ void store_stage(Smem32 &s, int buf, const Stage32 &s
   *reinterpret_cast<float4 *>(&s.B[buf][r][k + 4]) = make_float4(st.b[4], st.b[5], st.b[6], st.b[7]);
 }
 
-__device__ __forceinline__ void store_stage(Smem16 &s, int buf, const Stage16 &st) {
+// B is split here: plane q holds the rounding of what planes 0..q-1 left
+// (after the mode-2 scale, a power of two).
+template <int MODE>
+__device__ __forceinline__ void store_stage(SmemS<MODE> &s, int buf, const StageS<MODE> &st, float bscale) {
+  typedef typename Split<MODE>::T T;
+  typedef typename Split<MODE>::V V;
+  constexpr int NP = Split<MODE>::NP;
   const int r = gtid() >> 2, k = swz16(r, gtid() & 3);
-  *reinterpret_cast<bf16x8 *>(&s.Ah[buf][r][k]) = st.ah;
-  *reinterpret_cast<bf16x8 *>(&s.Al[buf][r][k]) = st.al;
-  bf16x8 h, l;
+#pragma unroll
+  for (int q = 0; q < NP; ++q) *reinterpret_cast<V *>(&s.A[buf][q][r][k]) = st.a[q];
+  V pl[NP];
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
-    const __bf16 x = (__bf16)st.b[i];
-    h[i] = x;
-    l[i] = (__bf16)(st.b[i] - (float)x);
+    float rem = MODE == 2 ? st.b[i] * bscale : st.b[i];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+      const T x = (T)rem;
+      pl[q][i] = x;
+      rem -= (float)x;
+    }
   }
-  *reinterpret_cast<bf16x8 *>(&s.Bh[buf][r][k]) = h;
-  *reinterpret_cast<bf16x8 *>(&s.Bl[buf][r][k]) = l;
+#pragma unroll
+  for (int q = 0; q < NP; ++q) *reinterpret_cast<V *>(&s.B[buf][q][r][k]) = pl[q];
 }
 
 // FP32: lane group g = lane / 16 takes k = 8 g + s in sub-step s for both
@@ -223,31 +286,36 @@ __device__ __forceinline__ void mma(const Smem32 &s, int buf, floatx4 (&acc)[2][
         acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[mi][t], b[ni][t], acc[mi][ni], 0, 0, 0);
 }
 
-// split-BF16: lane group g supplies the eight k values 8 g .. 8 g + 7 of its
-// row (A) and column (B); both operands use the same slots.
-__device__ __forceinline__ void mma(const Smem16 &s, int buf, floatx4 (&acc)[2][2]) {
+// Split modes: lane group g supplies the eight k values 8 g .. 8 g + 7 of
+// its row (A) and column (B); both operands use the same slots.  The plane
+// products are issued smallest first: (qa, qb) with qa + qb = NP - 1 down
+// to (0, 0).
+template <int MODE>
+__device__ __forceinline__ void mma(const SmemS<MODE> &s, int buf, floatx4 (&acc)[2][2]) {
+  typedef typename Split<MODE>::V V;
+  constexpr int NP = Split<MODE>::NP;
   const int lane = threadIdx.x & 63, wave = gtid() >> 6;
-  bf16x8 ah[2], al[2], bh[2], bl[2];
+  V a[2][NP], b[2][NP];
 #pragma unroll
   for (int mi = 0; mi < 2; ++mi) {
     const int r = (wave >> 1) * 32 + mi * 16 + (lane & 15);
-    ah[mi] = *reinterpret_cast<const bf16x8 *>(&s.Ah[buf][r][swz16(r, lane >> 4)]);
-    al[mi] = *reinterpret_cast<const bf16x8 *>(&s.Al[buf][r][swz16(r, lane >> 4)]);
+#pragma unroll
+    for (int q = 0; q < NP; ++q) a[mi][q] = *reinterpret_cast<const V *>(&s.A[buf][q][r][swz16(r, lane >> 4)]);
   }
 #pragma unroll
   for (int ni = 0; ni < 2; ++ni) {
     const int r = (wave & 1) * 32 + ni * 16 + (lane & 15);
-    bh[ni] = *reinterpret_cast<const bf16x8 *>(&s.Bh[buf][r][swz16(r, lane >> 4)]);
-    bl[ni] = *reinterpret_cast<const bf16x8 *>(&s.Bl[buf][r][swz16(r, lane >> 4)]);
+#pragma unroll
+    for (int q = 0; q < NP; ++q) b[ni][q] = *reinterpret_cast<const V *>(&s.B[buf][q][r][swz16(r, lane >> 4)]);
   }
 #pragma unroll
   for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-    for (int ni = 0; ni < 2; ++ni) {
-      acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[mi], bh[ni], acc[mi][ni], 0, 0, 0);
-      acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], bl[ni], acc[mi][ni], 0, 0, 0);
-      acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], bh[ni], acc[mi][ni], 0, 0, 0);
-    }
+    for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+      for (int ord = NP - 1; ord >= 0; --ord)
+#pragma unroll
+        for (int qa = ord; qa >= 0; --qa) acc[mi][ni] = mfma16(a[mi][qa], b[ni][ord - qa], acc[mi][ni]);
 }
 
 // acc += A[rows, k_begin..k_end) B[cols, k_begin..k_end)^T, the k steps
@@ -258,8 +326,8 @@ __device__ __forceinline__ void mma(const Smem16 &s, int buf, floatx4 (&acc)[2][
 // Every group passes the same number of block barriers (one per round),
 // also when it has no step left.
 template <class ST, class SM>
-__device__ __forceinline__ void run_k(const PGemm &p, int z, const float *B, float sb, float sub, int row0,
-                                      int col0, int k_begin, int k_end, SM &s, floatx4 (&acc)[2][2]) {
+__device__ __forceinline__ void run_k(const PGemm &p, int z, const float *B, float sb, float sub, float bscale,
+                                      int row0, int col0, int k_begin, int k_end, SM &s, floatx4 (&acc)[2][2]) {
   const int grp = threadIdx.x / kGroupThreads;
   const int steps = (k_end - k_begin + PK - 1) / PK;
   const int rounds = (steps + kGroups - 1) / kGroups;
@@ -269,19 +337,19 @@ __device__ __forceinline__ void run_k(const PGemm &p, int z, const float *B, flo
   if (have(0)) load_stage(p, z, B, sb, sub, row0, col0, kstep(0), even);
   if (have(1)) load_stage(p, z, B, sb, sub, row0, col0, kstep(1), odd);
   __syncthreads();  // the previous pass may still read buffer 0
-  if (have(0)) store_stage(s, 0, even);
+  if (have(0)) store_stage(s, 0, even, bscale);
   __syncthreads();
   for (int r = 0; r < rounds; r += 2) {
     // round r: buffer 0 holds step r, `odd` step r + 1, `even` is free
     if (have(r + 2)) load_stage(p, z, B, sb, sub, row0, col0, kstep(r + 2), even);
     if (have(r)) mma(s, 0, acc);
-    if (have(r + 1)) store_stage(s, 1, odd);
+    if (have(r + 1)) store_stage(s, 1, odd, bscale);
     __syncthreads();
     if (r + 1 >= rounds) break;
     // round r + 1: buffer 1 holds step r + 1, `even` step r + 2, `odd` is free
     if (have(r + 3)) load_stage(p, z, B, sb, sub, row0, col0, kstep(r + 3), odd);
     if (have(r + 1)) mma(s, 1, acc);
-    if (have(r + 2)) store_stage(s, 0, even);
+    if (have(r + 2)) store_stage(s, 0, even, bscale);
     __syncthreads();
   }
 }
@@ -312,10 +380,21 @@ __device__ __forceinline__ void reduce_groups(SM (&s)[kGroups], floatx4 (&acc)[2
   }
 }
 
-template <int BF16>
+// Power of two 2^e with |v| 2^e <= 2^kF16ScaleExp for every |v| <= m
+// (1 for m = 0 or not finite).
+__device__ __forceinline__ float f16_scale(float m) {
+  if (!(m > 0.f) || !isfinite(m)) return 1.f;
+  int e;
+  frexpf(m, &e);  // m < 2^e
+  e = kF16ScaleExp - e;
+  e = e < -100 ? -100 : (e > 100 ? 100 : e);
+  return ldexpf(1.f, e);
+}
+
+template <int MODE>
 __global__ void __launch_bounds__(kGroups * kGroupThreads) pgemm_kernel(PGemm p) {
-  typedef typename std::conditional<BF16 != 0, Stage16, Stage32>::type ST;
-  typedef typename std::conditional<BF16 != 0, Smem16, Smem32>::type SM;
+  typedef typename std::conditional<MODE != 0, StageS<MODE == 0 ? 1 : MODE>, Stage32>::type ST;
+  typedef typename std::conditional<MODE != 0, SmemS<MODE == 0 ? 1 : MODE>, Smem32>::type SM;
   static_assert(sizeof(SM) >= 16 * kGroupThreads * sizeof(float), "reduction buffer");
   __shared__ __attribute__((aligned(16))) SM s[kGroups];
   const int z = blockIdx.z;
@@ -327,6 +406,26 @@ __global__ void __launch_bounds__(kGroups * kGroupThreads) pgemm_kernel(PGemm p)
   const int lane = threadIdx.x & 63, wave = gtid() >> 6;
   const int row0 = blockIdx.y * PBM, col0 = blockIdx.x * PBN;
   const int roww = row0 + (wave >> 1) * 32, colw = col0 + (wave & 1) * 32;
+  // mode 2: the data operand's scale from the bound the previous stage left
+  float bscale = 1.f, out_scale = 1.f;
+  if (MODE == 2) {
+    float m = 0.f;
+    if (p.in_max_d) {
+      m = (float)*p.in_max_d;
+    } else {
+      __shared__ float wm[kGroups * kGroupThreads / 64];
+      for (int i = threadIdx.x; i < p.in_n; i += blockDim.x) m = fmaxf(m, p.in_max[i]);
+      for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+      if (lane == 0) wm[threadIdx.x >> 6] = m;
+      __syncthreads();
+      m = wm[0];
+#pragma unroll
+      for (int i = 1; i < kGroups * kGroupThreads / 64; ++i) m = fmaxf(m, wm[i]);
+    }
+    // folded B adds two values of at most m
+    bscale = f16_scale(m);
+    out_scale = p.a_inv[z] / bscale;
+  }
   floatx4 acc[2][2], odd[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -336,16 +435,17 @@ __global__ void __launch_bounds__(kGroups * kGroupThreads) pgemm_kernel(PGemm p)
       odd[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
     }
   if (p.out_fold) {
-    run_k<ST>(p, z, B, sb, sub, row0, col0, 0, p.khalf, s[grp], acc);
-    run_k<ST>(p, z, B, sb, sub, row0, col0, p.khalf, p.K, s[grp], odd);
+    run_k<ST>(p, z, B, sb, sub, bscale, row0, col0, 0, p.khalf, s[grp], acc);
+    run_k<ST>(p, z, B, sb, sub, bscale, row0, col0, p.khalf, p.K, s[grp], odd);
     reduce_groups(s, acc);
     reduce_groups(s, odd);
   } else {
-    run_k<ST>(p, z, B, sb, sub, row0, col0, 0, p.K, s[grp], acc);
+    run_k<ST>(p, z, B, sb, sub, bscale, row0, col0, 0, p.K, s[grp], acc);
     reduce_groups(s, acc);
   }
   if (grp != 0) return;
   const float se = p.se[z], so = p.so[z];
+  float wmax = 0.f;
 #pragma unroll
   for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
@@ -354,47 +454,97 @@ __global__ void __launch_bounds__(kGroups * kGroupThreads) pgemm_kernel(PGemm p)
       for (int r = 0; r < 4; ++r) {
         const int row = roww + mi * 16 + 4 * (lane >> 4) + r;
         const int col = colw + ni * 16 + (lane & 15);
-        const float e = acc[mi][ni][r], o = odd[mi][ni][r];
+        float e = acc[mi][ni][r], o = odd[mi][ni][r];
+        if (MODE == 2) {
+          e *= out_scale;
+          o *= out_scale;
+        }
         if (p.out_fold) {
           if (row < p.ho) {
-            C[(size_t)row * p.ldc + col] = e + o;
-            C[(size_t)(p.nfo - 1 - row) * p.ldc + col] = se * e + so * o;
+            const float v0 = e + o, v1 = se * e + so * o;
+            C[(size_t)row * p.ldc + col] = v0;
+            C[(size_t)(p.nfo - 1 - row) * p.ldc + col] = v1;
+            wmax = fmaxf(wmax, fmaxf(fabsf(v0), fabsf(v1)));
           }
         } else if (p.epi == 1) {
           const float a = p.wx2p[p.row_off[z] + row], b = p.wy2p[col];
           const bool dc = p.row_off[z] + row == 0 && col == 0;
-          C[(size_t)row * p.ldc + col] = (a < 0.f || b < 0.f || dc) ? 0.f : e / (a + b);
+          const float v = (a < 0.f || b < 0.f || dc) ? 0.f : e / (a + b);
+          C[(size_t)row * p.ldc + col] = v;
+          wmax = fmaxf(wmax, fabsf(v));
         } else {
           C[(size_t)row * p.ldc + col] = e;
+          wmax = fmaxf(wmax, fabsf(e));
         }
       }
+  if (MODE == 2 && p.out_max) {
+    for (int off = 32; off > 0; off >>= 1) wmax = fmaxf(wmax, __shfl_xor(wmax, off, 64));
+    if (lane == 0) {
+      const int blk = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+      p.out_max[blk * (kGroupThreads / 64) + wave] = wmax;
+    }
+  }
+}
+
+// Number of per-wave maxima a launch of this shape writes.
+static int wave_max_count(const PGemm &p, int batch) {
+  return (p.N / PBN) * (p.M / PBM) * batch * (kGroupThreads / 64);
 }
 
 static void gemm(Ctx &c, const PGemm &p, int batch) {
   dim3 grid(p.N / PBN, p.M / PBM, batch);
   const dim3 block(kGroups * kGroupThreads);
-  if (c.poisson->mode == 1) hipLaunchKernelGGL(pgemm_kernel<1>, grid, block, 0, c.stream, p);
-  else hipLaunchKernelGGL(pgemm_kernel<0>, grid, block, 0, c.stream, p);
+  switch (c.poisson->mode) {
+    case 1: hipLaunchKernelGGL(pgemm_kernel<1>, grid, block, 0, c.stream, p); break;
+    case 2: hipLaunchKernelGGL(pgemm_kernel<2>, grid, block, 0, c.stream, p); break;
+    case 3: hipLaunchKernelGGL(pgemm_kernel<3>, grid, block, 0, c.stream, p); break;
+    default: hipLaunchKernelGGL(pgemm_kernel<0>, grid, block, 0, c.stream, p); break;
+  }
 }
 
 static int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
-static bool upload(Mat &m, const std::vector<float> &h) {
-  std::vector<__bf16> hi(h.size()), lo(h.size());
-  for (size_t i = 0; i < h.size(); ++i) {
-    hi[i] = (__bf16)h[i];
-    lo[i] = (__bf16)(h[i] - (float)hi[i]);
+template <class T>
+static bool put(T *&d, const std::vector<T> &h) {
+  return hipMalloc(&d, h.size() * sizeof(T)) == hipSuccess &&
+         hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice) == hipSuccess;
+}
+
+// Uploads a constant matrix in the form the mode reads: FP32 always, bf16
+// planes (modes 1 and 3), or fp16 halves of the matrix times 2^f16_exp
+// (mode 2; the exponent puts the largest |entry| in [2^12, 2^13)).
+static bool upload(Mat &m, const std::vector<float> &h, int mode) {
+  if (!put(m.f, h)) return false;
+  if (mode == 1 || mode == 3) {
+    const int np = mode == 3 ? 3 : 2;
+    std::vector<float> rem(h);
+    for (int q = 0; q < np; ++q) {
+      std::vector<__bf16> pl(h.size());
+      for (size_t i = 0; i < h.size(); ++i) {
+        pl[i] = (__bf16)rem[i];
+        rem[i] -= (float)pl[i];
+      }
+      if (!put(m.bp[q], pl)) return false;
+    }
+  } else if (mode == 2) {
+    float mx = 0.f;
+    for (float v : h) mx = std::max(mx, std::fabs(v));
+    int e = 0;
+    if (mx > 0.f) std::frexp(mx, &e);
+    m.f16_exp = kF16ScaleExp - e;
+    std::vector<_Float16> hi(h.size()), lo(h.size());
+    for (size_t i = 0; i < h.size(); ++i) {
+      const float v = std::ldexp(h[i], m.f16_exp);
+      hi[i] = (_Float16)v;
+      lo[i] = (_Float16)(v - (float)hi[i]);
+    }
+    if (!put(m.hp[0], hi) || !put(m.hp[1], lo)) return false;
   }
-  return hipMalloc(&m.f, h.size() * 4) == hipSuccess &&
-         hipMemcpy(m.f, h.data(), h.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
-         hipMalloc(&m.hi, h.size() * 2) == hipSuccess &&
-         hipMemcpy(m.hi, hi.data(), h.size() * 2, hipMemcpyHostToDevice) == hipSuccess &&
-         hipMalloc(&m.lo, h.size() * 2) == hipSuccess &&
-         hipMemcpy(m.lo, lo.data(), h.size() * 2, hipMemcpyHostToDevice) == hipSuccess;
+  return true;
 }
 
 static void release(Mat &m) {
-  void *p[] = {m.f, m.hi, m.lo};
+  void *p[] = {m.f, m.bp[0], m.bp[1], m.bp[2], m.hp[0], m.hp[1]};
   for (void *q : p)
     if (q) (void)hipFree(q);
   m = Mat();
@@ -455,12 +605,21 @@ bool poisson_create(Ctx &c, std::string &err) {
     if (hipMalloc(&d, n * sizeof(float)) != hipSuccess) return false;
     return hipMemset(d, 0, n * sizeof(float)) == hipSuccess;
   };
-  bool ok = upload(q->FYA, FYA) && upload(q->FXA, FXA) && upload(q->CYT, CYT) && upload(q->SYT, SYT) &&
-            upload(q->CX2, CX2) && upload(q->SX2, SX2) && up(q->wx2p, wx2p) && up(q->wy2p, wy2p) &&
+  const int md = q->mode;
+  bool ok = upload(q->FYA, FYA, md) && upload(q->FXA, FXA, md) && upload(q->CYT, CYT, md) &&
+            upload(q->SYT, SYT, md) && upload(q->CX2, CX2, md) && upload(q->SX2, SX2, md) && up(q->wx2p, wx2p) &&
+            up(q->wy2p, wy2p) && (md != 2 || zeros(q->wmax, (size_t)3 * kMaxWaveMax)) &&
             zeros(q->T1t, (size_t)2 * Hyp * X) && zeros(q->spec, (size_t)2 * Hxp * 2 * Hyp) &&
             zeros(q->PQt, (size_t)2 * Y * 2 * Hxp) && zeros(c.rho, (size_t)X * Y) && zeros(c.phi, (size_t)X * Y) &&
             zeros(c.ex, (size_t)X * Y) && zeros(c.ey, (size_t)X * Y);
   if (!ok) err = "poisson_create: device allocation failed";
+  // per-wave maxima of the largest launch (stages 1 and 3: (X/64)(Hyp/64) and
+  // (2Hxp/64)(Hyp/64) blocks per batch, two batches, four waves each)
+  const long waves = 2L * 4 * (Hyp / 64) * std::max(X / 64, 2 * Hxp / 64);
+  if (ok && md == 2 && waves > kMaxWaveMax) {
+    err = "poisson_create: grid too large for the split-FP16 transforms";
+    ok = false;
+  }
   return ok;
 }
 
@@ -469,7 +628,7 @@ void poisson_destroy(Ctx &c) {
   if (!q) return;
   Mat *mats[] = {&q->FYA, &q->FXA, &q->CYT, &q->SYT, &q->CX2, &q->SX2};
   for (Mat *m : mats) release(*m);
-  float *ptrs[] = {q->wx2p, q->wy2p, q->T1t, q->spec, q->PQt};
+  float *ptrs[] = {q->wx2p, q->wy2p, q->T1t, q->spec, q->PQt, q->wmax};
   for (float *p : ptrs)
     if (p) (void)hipFree(p);
   delete q;
@@ -478,8 +637,13 @@ void poisson_destroy(Ctx &c) {
 
 static void set_a(PGemm &p, int z, const Mat &m, size_t off) {
   p.Af[z] = m.f + off;
-  p.Ah[z] = m.hi + off;
-  p.Al[z] = m.lo + off;
+  for (int q = 0; q < kMaxPlanes; ++q) p.Ap[z][q] = m.bp[q] ? m.bp[q] + off : nullptr;
+  if (m.hp[0]) {
+    p.Ap[z][0] = m.hp[0] + off;
+    p.Ap[z][1] = m.hp[1] + off;
+    p.Ap[z][2] = nullptr;
+  }
+  p.a_inv[z] = std::ldexp(1.f, -m.f16_exp);
 }
 
 void launch_poisson(Ctx &c, bool with_phi) {
@@ -501,6 +665,11 @@ void launch_poisson(Ctx &c, bool with_phi) {
   p.lda = Hyp; p.ldb = Y; p.ldc = X;
   p.M = Hyp; p.N = X; p.K = Hyp;
   p.nfb = c.ny; p.hb = q->hy;
+  // mode 2 scales: stage 1 from the largest charge density, later stages
+  // from the per-wave maxima of the stage before
+  p.in_max_d = &c.sc->rho_max;
+  p.out_max = q->wmax;
+  int n_prev = wave_max_count(p, 2);
   gemm(c, p, 2);
   // 2. spec[u][v] = sum_x F_X[u][x] T1t[v][x] / (wx^2 + wy^2)
   p = PGemm{};
@@ -519,6 +688,10 @@ void launch_poisson(Ctx &c, bool with_phi) {
   p.epi = 1;
   p.wx2p = q->wx2p;
   p.wy2p = q->wy2p;
+  p.in_max = q->wmax;
+  p.in_n = n_prev;
+  p.out_max = q->wmax + kMaxWaveMax;
+  n_prev = wave_max_count(p, 2);
   gemm(c, p, 2);
   // 3. Pt[y][u] = sum_v C_Y[v][y] spec[u][v] (mirror: e - o),
   //    Qt[y][u] = sum_v wy S_Y[v][y] spec[u][v] (mirror: o - e)
@@ -533,6 +706,10 @@ void launch_poisson(Ctx &c, bool with_phi) {
   p.lda = 2 * Hyp; p.ldb = 2 * Hyp; p.ldc = 2 * Hxp;
   p.M = Hyp; p.N = 2 * Hxp; p.K = 2 * Hyp; p.khalf = Hyp;
   p.out_fold = 1; p.nfo = c.ny; p.ho = q->hy;
+  p.in_max = q->wmax + kMaxWaveMax;
+  p.in_n = n_prev;
+  p.out_max = q->wmax + 2 * kMaxWaveMax;
+  n_prev = wave_max_count(p, 2);
   gemm(c, p, 2);
   // 4. Ex[x][y] = sum_u wx S_X[u][x] Pt[y][u] (mirror: o - e),
   //    Ey = sum_u C_X[u][x] Qt[y][u], phi = sum_u C_X[u][x] Pt[y][u] (e - o)
@@ -552,6 +729,8 @@ void launch_poisson(Ctx &c, bool with_phi) {
   p.lda = 2 * Hxp; p.ldb = 2 * Hxp; p.ldc = Y;
   p.M = Hxp; p.N = Y; p.K = 2 * Hxp; p.khalf = Hxp;
   p.out_fold = 1; p.nfo = c.nx; p.ho = q->hx;
+  p.in_max = q->wmax + 2 * kMaxWaveMax;
+  p.in_n = n_prev;
   gemm(c, p, with_phi ? 3 : 2);
 }
 

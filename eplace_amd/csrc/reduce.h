@@ -33,6 +33,27 @@ __device__ __forceinline__ double block_sum(double v) {
   return r;
 }
 
+// Maximum over the block (blockDim.x a multiple of 64, at most 1024); the
+// result is returned to every thread.
+__device__ __forceinline__ double block_max(double v) {
+  __shared__ double red[16];
+  __shared__ double total;
+  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    int nw = (blockDim.x + 63) >> 6;
+    for (int i = 1; i < nw; ++i) m = fmax(m, red[i]);
+    total = m;
+  }
+  __syncthreads();
+  double r = total;
+  __syncthreads();
+  return r;
+}
+
 __device__ __forceinline__ double block_sum_partials(const double *__restrict__ p, int n) {
   double s = 0;
 #pragma unroll 4

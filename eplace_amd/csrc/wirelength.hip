@@ -326,9 +326,10 @@ __global__ void __launch_bounds__(kBlock) wl_group_kernel(WlClasses cls, LanePin
 
 // Sums the per-block HPWL/WL partials in a fixed order; optionally applies
 // updatePenaltyFactor (eplace.cpp:846-883) with FP64 HPWL and lambda
-// (lambda_update 1), times the controller's factor (lambda_update 2).
+// (lambda_update 1), times the controller's factor (lambda_update 2).  The
+// WA value is stored only when the wiring pass computed it (with_value).
 __global__ void __launch_bounds__(kFinish) reduce_wl_kernel(int nslots, double *partials, DevScalars *sc,
-                                                            int lambda_update) {
+                                                            int lambda_update, int with_value) {
   if (sc->done) return;
   const double *const src[2] = {partials + kPartialWL, partials + kPartialWL + 1};
   double sum[2];
@@ -336,7 +337,7 @@ __global__ void __launch_bounds__(kFinish) reduce_wl_kernel(int nslots, double *
   const double hp = sum[0], wv = sum[1];
   if (threadIdx.x == 0) {
     sc->hpwl = hp;
-    sc->wl_value = wv;
+    if (with_value) sc->wl_value = wv;
     if (lambda_update) {
       double delta = hp - sc->last_hpwl;
       double mult = delta < 0.0 ? 1.05 : pow(1.05, -delta / 350000.0 + 1.0);
@@ -397,10 +398,11 @@ void launch_wirelength(Ctx &c, bool gradient, hipStream_t st) {
   }
 }
 
-void launch_reduce_wirelength(Ctx &c, int lambda_update, hipStream_t st) {
+void launch_reduce_wirelength(Ctx &c, int lambda_update, hipStream_t st, bool with_value) {
   if (!st) st = c.stream;
   int n = g_small_blocks(c) + (c.num_large ? g_large_blocks(c) : 0);
-  hipLaunchKernelGGL(reduce_wl_kernel, dim3(1), dim3(kFinish), 0, st, n, c.partials, c.sc, lambda_update);
+  hipLaunchKernelGGL(reduce_wl_kernel, dim3(1), dim3(kFinish), 0, st, n, c.partials, c.sc, lambda_update,
+                     with_value ? 1 : 0);
 }
 
 }  // namespace ep

@@ -170,11 +170,6 @@ class EPlacer2D:
         s = self.stats()
         return f"Overflow: {s['overflow']}\npenalty factor: {s['lambda']:.6f}\nHPWL: {s['last_hpwl']}\n"
 
-    def calcHPWL(self) -> float:
-        v = C.c_double()
-        self._check(self.lib.ep_hpwl(self.h, C.byref(v)))
-        return v.value
-
     # -- raw arrays (parity tests) -----------------------------------------
     def read(self, which: str) -> np.ndarray:
         i = self.info()

@@ -31,7 +31,7 @@ def training_design(i: int, cells: int):
                      num_macros=macros, name=f"train{i}")
 
 
-def trajectories(n: int, lo: int, hi: int, poisson_mode: int = 1, verbose: bool = True):
+def trajectories(n: int, lo: int, hi: int, poisson_mode: int = 2, verbose: bool = True):
     """Plain backtracking logs of n training designs."""
     rng = np.random.default_rng(7)
     logs, info = [], []
@@ -55,7 +55,7 @@ def samples(logs, lookahead: float = 1.0):
     return np.concatenate([p[0] for p in pairs]), np.concatenate([p[1] for p in pairs])
 
 
-def compare(ctrl: Controller, design, bins=0, poisson_mode: int = 1, target_overflow: float = 0.10):
+def compare(ctrl: Controller, design, bins=0, poisson_mode: int = 2, target_overflow: float = 0.10):
     """Plain backtracking vs learned on one design, both from the same start."""
     out = {}
     for method in ("bktrk", "learned"):

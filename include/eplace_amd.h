@@ -63,7 +63,9 @@ typedef struct ep_params {
   uint32_t filler_seed;     /* rand() seed for filler placement; 1 = reference */
   const float *filler_xy;   /* optional filler centres, overrides the seed */
   int32_t filler_count;     /* -1: reference count (eplace.cpp:167) */
-  int32_t poisson_mode;     /* 0 FP32 MFMA, 1 split-BF16 (3 products) MFMA */
+  int32_t poisson_mode;     /* 0 FP32 MFMA; 1 split-BF16, 2 planes, 3 products; 2 split-FP16
+                               with power-of-two operand scales, 3 products; 3 split-BF16,
+                               3 planes, 6 products (all accumulate in FP32) */
   int32_t device;           /* HIP device ordinal */
 } ep_params;
 

@@ -85,4 +85,4 @@ def test_create_rejects_odd_bins_and_unknown_poisson_mode():
     with pytest.raises(L.EPError, match="even"):
         EPlacer2D(d, 1.0, bins=(15, 16))
     with pytest.raises(L.EPError, match="poisson_mode"):
-        EPlacer2D(d, 1.0, poisson_mode=2)
+        EPlacer2D(d, 1.0, poisson_mode=4)

@@ -1,13 +1,14 @@
 """The workload configurations at their full sizes on the GPU, checked
 through properties that do not need a reference run of the same size:
 
-* 200K std cells, 512x512 bins, FP32 transforms: one gradient evaluation
-  against the FP64 oracle on the same positions (fixed-point density maps
-  bit-exact, fields and gradients within 1e-5 relative);
-* 500K std cells + 60 movable macros, 1024x1024 bins, split-BF16
+* 200K std cells, 512x512 bins, FP32 and split-FP16 transforms: one
+  gradient evaluation against the FP64 oracle on the same positions
+  (fixed-point density maps bit-exact, fields and gradients within 1e-5
+  relative);
+* 500K std cells + 60 movable macros, 1024x1024 bins, split-FP16
   transforms: mGP -> macro legalisation -> FILLERONLY -> cGP -> Abacus
   reaches the stage targets and legal standard cells;
-* 2M std cells, 2048x2048 bins, split-BF16 transforms: the learned
+* 2M std cells, 2048x2048 bins, split-FP16 transforms: the learned
   controller and plain backtracking both reach overflow 0.10, the learned
   run in fewer gradient evaluations and within 1% of the plain HPWL.
   (With bins 1.34 rows high the overflow of plain backtracking levels off
@@ -48,10 +49,11 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / (n if n > 0 else 1.0))
 
 
-def test_200k_cells_512_bins_against_fp64_oracle():
-    say("[config] 200K cells, 512x512 bins, FP32 transforms vs the FP64 oracle")
+@pytest.mark.parametrize("mode", [0, 2], ids=["fp32", "split-fp16"])
+def test_200k_cells_512_bins_against_fp64_oracle(mode):
+    say(f"[config] 200K cells, 512x512 bins, poisson_mode {mode} vs the FP64 oracle")
     d = synthetic(200000, 210000, seed=21, num_io=1000, util=0.7, name="cells200k")
-    gp = EPlacer2D(d, 1.0, bins=512, poisson_mode=0)
+    gp = EPlacer2D(d, 1.0, bins=512, poisson_mode=mode)
     try:
         orc = Oracle(d, 1.0, mode="fp64", bins=512, exp="fast", wl="WA")
         orc.set("positions", orc.get("positions"))
@@ -79,7 +81,7 @@ def run_mixed(cells, nets, macros, bins, util, seed, legal, **kw):
     d = synthetic(cells, nets, seed=seed, num_io=1000, util=util, num_macros=macros, name=f"mixed{cells}")
     say(f"  design: {d.num_nodes} nodes, {d.num_pins} pins ({time.perf_counter() - t:.1f}s)")
     t = time.perf_counter()
-    out = place(d, 1.0, qp=False, legal=legal, bins=bins, poisson_mode=1, verbose=True, **kw)
+    out = place(d, 1.0, qp=False, legal=legal, bins=bins, poisson_mode=2, verbose=True, **kw)
     ext = macro_overlap_extent(d, out["positions"])
     say(f"  flow {time.perf_counter() - t:.1f}s: mGP {out['mgp']['iterations']} it (overflow "
         f"{out['mgp']['final_overflow']:.4f}), mLG overlap {out['mlg']['final_overlap']} "
@@ -109,7 +111,7 @@ def test_2m_cells_learned_controller_against_plain():
     from eplace_amd.controller import Controller
     t = time.perf_counter()
     d = synthetic(2000000, 2100000, seed=5, num_io=1000, util=0.8, name="cells2m")
-    gp = EPlacer2D(d, 1.0, bins=2048, poisson_mode=1)
+    gp = EPlacer2D(d, 1.0, bins=2048, poisson_mode=2)
     say(f"  design and engine: {d.num_pins} pins, {gp.info()['num_fillers']} fillers "
         f"({time.perf_counter() - t:.1f}s)")
     try:

@@ -111,7 +111,7 @@ def test_trained_controller_gain_on_1m_cells():
     ctrl = Controller.load()
     res = {}
     for method in ("bktrk", "learned"):
-        gp = EPlacer2D(d, 1.0, poisson_mode=1)
+        gp = EPlacer2D(d, 1.0, poisson_mode=2)
         gp.initialization()
         res[method] = EplaceNesterovOpt(gp, method=method, controller=ctrl if method == "learned" else None,
                                         log=False).opt()

@@ -21,7 +21,7 @@ def say(msg):
 def test_gp_then_abacus_is_legal(cells):
     say(f"[flow] {cells} cells: GP -> Abacus -> check")
     d = synthetic(cells, int(cells * 1.05), seed=31, num_io=200, util=0.7, name="flow")
-    gp = EPlacer2D(d, 1.0, poisson_mode=1)
+    gp = EPlacer2D(d, 1.0, poisson_mode=2)
     gp.initialization()
     r = EplaceNesterovOpt(gp, method="bktrk").opt()
     assert r["final_overflow"] < 0.1
