@@ -91,9 +91,10 @@ typedef struct ep_stats {
 /* Columns of one optimiser log row (written at the end of iteration k). */
 enum {
   EP_LOG_ITER = 0,          /* k */
-  EP_LOG_TAU = 1,           /* overflow at the accepted point y_{k+1} */
-  EP_LOG_LAMBDA = 2,        /* lambda after the update of iteration k */
-  EP_LOG_HPWL = 3,          /* HPWL at y_{k+1} */
+  EP_LOG_TAU = 1,           /* overflow at the accepted point y_{k+1} (vanilla: at y_k) */
+  EP_LOG_LAMBDA = 2,        /* lambda after the update of iteration k (vanilla: of iteration
+                               k-1; its update at y_{k+1} runs inside the next evaluation) */
+  EP_LOG_HPWL = 3,          /* HPWL at y_{k+1} (vanilla: at y_k) */
   EP_LOG_STEP = 4,          /* 1/L' measured at the accepted trial (backtracking) */
   EP_LOG_INV_GAMMA = 5,     /* 1/gamma_x */
   EP_LOG_EVALS = 6,         /* full evaluations so far */

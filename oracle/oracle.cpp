@@ -255,6 +255,9 @@ struct Placer : PlacerBase {
   int iter = 0;
   T last_step = 0;
   int bktrk_count = 0;
+  // backtracking acceptance factor (BKTRK_EPS, nesterov.hpp:37); settable only
+  // so that tests can plant a wrong value and see the trajectory check fail
+  double bktrk_eps = 0.95;
 
   // ---------------------------------------------------------------------
   void load(const orc_design *d, float target_density) {
@@ -1075,7 +1078,9 @@ struct Placer : PlacerBase {
         total_gradient_update(fixed_density, shift);
         get_gradient(ngrad);
         T nstep = (T)1 / lipschitz(nref, cur_ref, ngrad, cur_grad);
-        if ((T)0.95 * step <= nstep) {
+        // BKTRK_EPS is a double literal: the product and the comparison are
+        // double in the reference too (nesterov.hpp:355)
+        if (bktrk_eps * (double)step <= (double)nstep) {
           step = nstep;
           break;
         }
@@ -1185,6 +1190,7 @@ void orc_set_scalar(void *p, const char *name, double v) {
     else if (n == "stage") P->stage = (int)v;
     else if (n == "poisson_fast") P->poisson_fast = (int)v;
     else if (n == "exp_mode") P->exp_mode = (int)v;
+    else if (n == "bktrk_eps") P->bktrk_eps = v;
   });
 }
 

@@ -333,10 +333,35 @@ def test_nesterov_run_matches_oracle(method, case, tol):
     assert abs(r["final_hpwl"] - orc.hpwl()) <= tol * orc.hpwl()
 
 
+def oracle_rows(orc, shift, method, K):
+    rows = []
+    for _ in range(K):
+        orc.run("opt_step", 1 if method == "bktrk" else 0, shift)
+        i = orc.info()
+        rows.append(dict(tau=i["overflow"], lam=i["lambda"], hpwl=i["last_hpwl"], step=i["last_step"],
+                         bk=i["bktrk_count"]))
+    return rows
+
+
+# Per-row bars of the GPU against the FP64 oracle over the first K
+# iterations (same fixed-point density on both sides; the field differs by
+# the FP32 transform's ~1e-6, which the iteration amplifies).  Measured on
+# MI355X: at most 2.3e-7 for overflow, lambda and HPWL and 9.1e-7 for the
+# step on both designs and methods.
+TRAJ_TOL = dict(tau=1e-6, lam=1e-6, hpwl=1e-6, step=5e-6)
+
+
 @pytest.mark.parametrize("case", ["c1_16", "c2_small_64"])
-def test_vanilla_trajectory_first_iterations(case):
-    """Step-by-step agreement before the runs separate (chaotic dynamics)."""
-    say(f"[trajectory] case={case}")
+@pytest.mark.parametrize("method", ["vanilla", "bktrk"])
+def test_trajectory_first_iterations(case, method):
+    """Row-by-row agreement with the oracle for the first 12 iterations
+    (before the runs separate: the dynamics are chaotic): overflow, lambda,
+    HPWL and the Lipschitz step to about six digits, and under backtracking
+    the same rejected trials at the same iterations (nesterov.hpp:310-369,
+    375-415).  Vanilla rows log lambda and HPWL one iteration late (the
+    penalty update at y_{k+1} runs inside the next evaluation), so those two
+    are compared with the oracle's previous row."""
+    say(f"[trajectory] case={case} method={method}")
     d, gp, orc = make(case)
     gp.initialization()
     shift = gp.info()["fixed_shift"]
@@ -344,21 +369,28 @@ def test_vanilla_trajectory_first_iterations(case):
     orc.run("penalty_init")
     orc.run("opt_init")
     K = 12
-    otraj = []
-    for k in range(K):
-        orc.run("opt_step", 0, shift)
-        i = orc.info()
-        otraj.append((i["overflow"], i["lambda"], i["last_hpwl"], i["last_step"]))
-    opt = EplaceNesterovOpt(gp, max_iterations=K - 1)
-    r = opt.opt()
+    otraj = oracle_rows(orc, shift, method, K)
+    opt = EplaceNesterovOpt(gp, method=method, max_iterations=K - 1)
+    opt.opt()
     log = opt.log
-    for k in range(min(K, len(log))):
-        gl = log[k]
-        say(f"  k={k:2d} tau gpu {gl[1]:.6f} orc {otraj[k][0]:.6f} | step gpu {gl[4]:.5g} orc {otraj[k][3]:.5g}"
-            f" | lam gpu {gl[2]:.5g} | hpwl(y_k) gpu {gl[3]:.6g}")
-    for k in range(4):
-        assert abs(log[k][1] - otraj[k][0]) <= 1e-3 * otraj[k][0]
-        assert abs(log[k][4] - otraj[k][3]) <= 2e-2 * otraj[k][3]
+    assert len(log) >= K
+    worst = {k: 0.0 for k in TRAJ_TOL}
+    for k in range(K):
+        gl, o = log[k], dict(otraj[k])
+        if method == "vanilla":
+            if k == 0:
+                continue
+            o["lam"], o["hpwl"] = otraj[k - 1]["lam"], otraj[k - 1]["hpwl"]
+        g = dict(tau=gl[1], lam=gl[2], hpwl=gl[3], step=gl[4])
+        say(f"  k={k:2d} tau {g['tau']:.7f}/{o['tau']:.7f} step {g['step']:.6g}/{o['step']:.6g} "
+            f"lam {g['lam']:.6g}/{o['lam']:.6g} hpwl {g['hpwl']:.7g}/{o['hpwl']:.7g} bk {int(gl[7])}/{o['bk']}")
+        for nm in TRAJ_TOL:
+            worst[nm] = max(worst[nm], abs(g[nm] - o[nm]) / abs(o[nm]))
+        if method == "bktrk":
+            assert int(gl[7]) == o["bk"], (k, int(gl[7]), o["bk"])
+    say("  worst relative differences: " + ", ".join(f"{k} {v:.1e}" for k, v in worst.items()))
+    for nm, tol in TRAJ_TOL.items():
+        assert worst[nm] <= tol, (nm, worst[nm])
 
 
 def test_hpwl_keeps_wirelength_value():

@@ -122,3 +122,55 @@ def test_poisson_closed_form_vs_reference_fft(key, method):
     assert rel(ex, z[key + "_ex"]) < 2e-6
     assert rel(ey, z[key + "_ey"]) < 2e-6
     assert rel(phi, z[key + "_phi"]) < 2e-6
+
+
+def bktrk_rows(o, k):
+    rows = []
+    for _ in range(k):
+        o.run("opt_step", 1, -1)
+        i = o.info()
+        rows.append((i["overflow"], i["lambda"], i["last_hpwl"], i["invgx"], i["bktrk_count"]))
+    return rows
+
+
+def bktrk_oracle(eps=None):
+    g = Golden("smoke_bktrk")
+    d, o = oracle_at_init(g)
+    if eps is not None:
+        o.set_scalar("bktrk_eps", eps)
+    o.run("total_gradient_update", 0, -1)
+    o.run("penalty_init")
+    o.run("opt_init")
+    return g, o
+
+
+def test_bktrk_trajectory_per_iteration():
+    """The reference's -bktrk run on the smoke design, row by row
+    (nesterov.hpp:310-369; trajectory columns k, tau, lambda, HPWL,
+    1/gamma_x, 1/gamma_y, a_k).  The first nine iterations hold five
+    backtracks; overflow, HPWL and 1/gamma equal the reference's float
+    values exactly there, lambda to its float rounding.  Later rows carry
+    the FFT rounding difference (~1e-6), amplified by the dynamics."""
+    g, o = bktrk_oracle()
+    T = g["trajectory"]
+    rows = bktrk_rows(o, 20)
+    assert rows[8][4] == 5  # backtracks within the exact rows
+    for k, (tau, lam, hp, igx, _) in enumerate(rows):
+        t = T[k]
+        if k < 9:
+            assert np.float32(tau) == np.float32(t[1]), k
+            assert np.float32(hp) == np.float32(t[3]), k
+            assert np.float32(igx) == np.float32(t[4]), k
+            assert abs(lam - t[2]) <= 2e-7 * t[2], k
+        else:
+            for a, b in ((tau, t[1]), (hp, t[3]), (igx, t[4]), (lam, t[2])):
+                assert abs(a - b) <= 3e-5 * abs(b), (k, a, b)
+
+
+def test_bktrk_trajectory_check_catches_wrong_acceptance():
+    """The same comparison with the acceptance factor planted wrong in the
+    oracle (1.0 instead of BKTRK_EPS = 0.95) fails within the exact rows."""
+    g, o = bktrk_oracle(eps=1.0)
+    T = g["trajectory"]
+    rows = bktrk_rows(o, 9)
+    assert any(np.float32(r[0]) != np.float32(T[k][1]) for k, r in enumerate(rows))
