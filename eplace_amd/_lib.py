@@ -17,6 +17,7 @@ EP_ERR_ARG, EP_ERR_HIP, EP_ERR_STATE, EP_ERR_NUMERIC, EP_ERR_NOMEM = 1, 2, 3, 4,
 EP_STAGE_MGP, EP_STAGE_FILLERONLY, EP_STAGE_CGP = 0, 1, 2
 
 EP_LOG_COLS = 24
+EP_QP_NOALLOC_MAX_ROWS = 1 << 20
 EP_LOG_ITER, EP_LOG_TAU, EP_LOG_LAMBDA, EP_LOG_HPWL, EP_LOG_STEP, EP_LOG_INV_GAMMA, EP_LOG_EVALS, \
     EP_LOG_BACKTRACKS, EP_LOG_FEATURES = range(9)
 EP_LOG_STEP_PROP, EP_LOG_STEP_USED, EP_LOG_LAMBDA_BEFORE, EP_LOG_FALLBACKS = 20, 21, 22, 23
@@ -164,6 +165,8 @@ SYMBOLS = {
     "ep_qplace": ([C.c_void_p] * 5, C.c_int),
     "ep_qp_matrix": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p], C.c_int),
     "ep_qp_solve": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_float, C.c_int32, C.c_void_p, C.c_void_p], C.c_int),
+    "ep_qp_solve_noalloc": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_float, C.c_int32, C.c_void_p, C.c_void_p,
+                             C.c_void_p], C.c_int),
     "ep_mlg_last_error": ([], C.c_char_p),
     "ep_legalize_macros": ([C.c_void_p] * 5, C.c_int),
     "ep_macro_legalize": ([C.c_void_p] * 4, C.c_int),

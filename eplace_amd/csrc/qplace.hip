@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "ep_internal.h"
@@ -599,8 +600,9 @@ struct Solver {
     nblocks = std::min(kQpMaxBlocks, grid_for(n, kQpThreads / kQpLanes));
   }
 
-  // Solves from the guess in S.x; returns the iterations run per direction.
-  void solve(hipStream_t st, int max_iter, float tol, std::vector<QpSolveScalars> &out) {
+  // Solves from the guess in S.x; out[ndim] receives the final scalars of
+  // each direction (iterations, error).
+  void solve(hipStream_t st, int max_iter, float tol, QpSolveScalars *out) {
     const int n = S.n;
     const dim3 g(nblocks, ndim), gu(std::min(kQpMaxBlocks, grid_for(n, kQpThreads)), ndim);
     const int64_t m = (int64_t)n * ndim;
@@ -611,7 +613,7 @@ struct Solver {
     // Restarts reset the count once, so a direction may run up to
     // 2 max_iter iterations; the host checks every kCheck iterations.
     constexpr int kCheck = 16;
-    out.assign(ndim, QpSolveScalars{});
+    for (int d = 0; d < ndim; ++d) out[d] = QpSolveScalars{};
     for (int it = 0;; ++it) {
       hipLaunchKernelGGL(qp_residual_kernel, g, dim3(kQpThreads), 0, st, S, sc, 1, partials);
       hipLaunchKernelGGL(qp_restart_finish_kernel, dim3(ndim), dim3(kFinish), 0, st, sc, partials, nblocks);
@@ -624,15 +626,15 @@ struct Solver {
       hipLaunchKernelGGL(qp_update_kernel, gu, dim3(kQpThreads), 0, st, S, sc, partials);
       hipLaunchKernelGGL(qp_iter_finish_kernel, dim3(ndim), dim3(kFinish), 0, st, sc, partials, gu.x);
       if ((it + 1) % kCheck == 0 || it + 1 >= 2 * max_iter) {
-        QP_CHECK(hipMemcpyAsync(out.data(), sc, ndim * sizeof(QpSolveScalars), hipMemcpyDeviceToHost, st));
+        QP_CHECK(hipMemcpyAsync(out, sc, ndim * sizeof(QpSolveScalars), hipMemcpyDeviceToHost, st));
         QP_CHECK(hipStreamSynchronize(st));
         bool any = false;
-        for (auto &s : out) any |= s.active != 0;
+        for (int d = 0; d < ndim; ++d) any |= out[d].active != 0;
         if (!any || it + 1 >= 2 * max_iter) break;
       }
     }
     hipLaunchKernelGGL(qp_solve_end_kernel, dim3(grid_for(m, 256)), dim3(256), 0, st, n, ndim, sc, S.x);
-    QP_CHECK(hipMemcpyAsync(out.data(), sc, ndim * sizeof(QpSolveScalars), hipMemcpyDeviceToHost, st));
+    QP_CHECK(hipMemcpyAsync(out, sc, ndim * sizeof(QpSolveScalars), hipMemcpyDeviceToHost, st));
     QP_CHECK(hipStreamSynchronize(st));
   }
 };
@@ -780,6 +782,70 @@ struct QpCtx {
   }
 };
 
+// ---------------------------------------------------------------------------
+// Work space of ep_qp_solve_noalloc: the six BiCGSTAB work vectors (r, r0,
+// p, v, s, t) of up to EP_QP_NOALLOC_MAX_ROWS rows, the per-block partial
+// sums, the per-direction scalars and a validation flag, in one static
+// device array.  Each device that runs the library holds its own copy until
+// the process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns on that device.
+constexpr size_t kNaRows = EP_QP_NOALLOC_MAX_ROWS;
+constexpr size_t kNaVecBytes = kNaRows * sizeof(float);
+constexpr size_t kNaPartialOff = 6 * kNaVecBytes;
+constexpr size_t kNaScalarOff = kNaPartialOff + (size_t)2 * 2 * kQpMaxBlocks * sizeof(double);
+constexpr size_t kNaFlagOff = kNaScalarOff + 256;
+constexpr size_t kNaBytes = kNaFlagOff + 256;
+static_assert(2 * sizeof(QpSolveScalars) <= 256, "scalar slot");
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mu;
+unsigned char *g_na_ptr[kNaMaxDevices] = {};
+
+// Device address of `scratch` on `device` (the current device), looked up
+// once per device.
+unsigned char *scratch_on(int device) {
+  if (device < 0 || device >= kNaMaxDevices) throw std::string("device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_na_mu);
+  if (!g_na_ptr[device]) {
+    void *p = nullptr;
+    QP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw std::string("hipGetSymbolAddress returned no address for the work space");
+    g_na_ptr[device] = static_cast<unsigned char *>(p);
+  }
+  return g_na_ptr[device];
+}
+
+__global__ void qp_flag_clear_kernel(int *flag) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *flag = 0;
+}
+
+// Checks a device CSR before the solve reads it through its column
+// indices: rowptr[0] = 0, rowptr[rows] = nnz, rows non-decreasing, columns in
+// [0, n).  Reads only rowptr[0..rows] and col[k] for 0 <= k < nnz.  Bits of
+// *flag: 1 bad row bounds, 2 decreasing rows, 4 column out of range.
+__global__ void qp_check_system_kernel(int rows, int n, int64_t nnz, const int32_t *__restrict__ rowptr,
+                                       const int32_t *__restrict__ col, int *flag) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  const int64_t a = rowptr[r], b = rowptr[r + 1];
+  int bad = 0;
+  if (r == 0 && a != 0) bad |= 1;
+  if (r == rows - 1 && b != nnz) bad |= 1;
+  if (b < a) {
+    bad |= 2;
+  } else if (a < 0 || b > nnz) {
+    bad |= 1;
+  } else {
+    for (int64_t k = a; k < b; ++k)
+      if (col[k] < 0 || col[k] >= n) {
+        bad |= 4;
+        break;
+      }
+  }
+  if (bad) atomicOr(flag, bad);
+}
+
 thread_local std::string g_qp_err;
 
 int qp_fail(const std::string &m, int code) {
@@ -824,7 +890,7 @@ int ep_qplace(const ep_design *design, const ep_qp_params *params, float *node_x
     ep_qp_stats s{};
     s.initial_hpwl = h0;
     int it = 0;
-    std::vector<QpSolveScalars> res;
+    QpSolveScalars res[2];
     for (;; ++it) {
       const int64_t nnz = c.assemble();
       (void)nnz;
@@ -934,7 +1000,7 @@ int ep_qp_solve(const ep_qp_system *sys, float *x, int32_t max_iterations, float
     sv.S.diag = sys->diag ? dupload(owned, sys->diag, rows, st) : nullptr;
     sv.S.b = dupload(owned, sys->b, rows, st);
     QP_CHECK(hipMemcpyAsync(sv.S.x, x, rows * 4, hipMemcpyHostToDevice, st));
-    std::vector<QpSolveScalars> res;
+    QpSolveScalars res[2];
     sv.solve(st, max_iterations, tolerance, res);
     QP_CHECK(hipMemcpyAsync(x, sv.S.x, rows * 4, hipMemcpyDeviceToHost, st));
     QP_CHECK(hipStreamSynchronize(st));
@@ -947,6 +1013,65 @@ int ep_qp_solve(const ep_qp_system *sys, float *x, int32_t max_iterations, float
     return qp_fail(e, EP_ERR_HIP);
   } catch (const std::bad_alloc &) {
     return qp_fail("host allocation failed", EP_ERR_NOMEM);
+  }
+}
+
+int ep_qp_solve_noalloc(const ep_qp_system *sys, float *x, int32_t max_iterations, float tolerance, int32_t device,
+                        void *stream, float *error, int32_t *iterations) {
+  g_qp_err.clear();
+  if (!sys || !x || sys->n <= 0 || sys->ndim < 1 || sys->ndim > 2 || !sys->rowptr || !sys->b ||
+      (sys->nnz > 0 && (!sys->col || !sys->val)) || sys->nnz < 0 || max_iterations < 0)
+    return qp_fail("bad arguments", EP_ERR_ARG);
+  const int n = sys->n, nd = sys->ndim;
+  const int64_t rows = (int64_t)n * nd;
+  if (rows > (int64_t)EP_QP_NOALLOC_MAX_ROWS)
+    return qp_fail("system of " + std::to_string(rows) + " rows exceeds the allocation-free work space of " +
+                       std::to_string((long long)EP_QP_NOALLOC_MAX_ROWS) + " rows",
+                   EP_ERR_ARG);
+  if (sys->nnz > (int64_t)INT32_MAX) return qp_fail("too many entries", EP_ERR_ARG);
+  try {
+    QP_CHECK(hipSetDevice(device));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    unsigned char *ws = scratch_on(device);
+    int *flag = reinterpret_cast<int *>(ws + kNaFlagOff);
+    hipLaunchKernelGGL(qp_flag_clear_kernel, dim3(1), dim3(64), 0, st, flag);
+    hipLaunchKernelGGL(qp_check_system_kernel, dim3(grid_for(rows, 256)), dim3(256), 0, st, (int)rows, n, sys->nnz,
+                       sys->rowptr, sys->col, flag);
+    int bad = 0;
+    QP_CHECK(hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, st));
+    QP_CHECK(hipStreamSynchronize(st));
+    if (bad & 4) return qp_fail("column index out of range", EP_ERR_ARG);
+    if (bad & 2) return qp_fail("rowptr not monotone", EP_ERR_ARG);
+    if (bad) return qp_fail("rowptr does not match nnz", EP_ERR_ARG);
+    Solver sv;
+    sv.ndim = nd;
+    sv.nblocks = std::min(kQpMaxBlocks, grid_for(n, kQpThreads / kQpLanes));
+    sv.S.n = n;
+    sv.S.x = x;
+    float *vec[6];
+    for (int i = 0; i < 6; ++i) vec[i] = reinterpret_cast<float *>(ws + i * kNaVecBytes);
+    sv.S.r = vec[0];
+    sv.S.r0 = vec[1];
+    sv.S.p = vec[2];
+    sv.S.v = vec[3];
+    sv.S.s = vec[4];
+    sv.S.t = vec[5];
+    sv.partials = reinterpret_cast<double *>(ws + kNaPartialOff);
+    sv.sc = reinterpret_cast<QpSolveScalars *>(ws + kNaScalarOff);
+    sv.S.rowptr = sys->rowptr;
+    sv.S.col = sys->col;
+    sv.S.val = sys->val;
+    sv.S.diag = sys->diag;
+    sv.S.b = sys->b;
+    QpSolveScalars res[2];
+    sv.solve(st, max_iterations, tolerance, res);
+    for (int d = 0; d < nd; ++d) {
+      if (error) error[d] = res[d].error;
+      if (iterations) iterations[d] = res[d].iter;
+    }
+    return EP_OK;
+  } catch (const std::string &e) {
+    return qp_fail(e, EP_ERR_HIP);
   }
 }
 

@@ -90,3 +90,21 @@ def qp_solve(rowptr, col, val, b, x0, diag=None, ndim: int = 1, max_iterations: 
     _err(lib, lib.ep_qp_solve(C.byref(sys_), x.ctypes.data, int(max_iterations), float(tolerance), int(device),
                               err.ctypes.data, its.ctypes.data))
     return x, err, its
+
+
+def qp_solve_noalloc(n: int, ndim: int, nnz: int, rowptr: int, col: int, val: int, b: int, x: int,
+                     diag: int = 0, max_iterations: int = 100, tolerance: float = FLOAT_EPS, device: int = 0,
+                     stream: int = 0):
+    """ep_qp_solve_noalloc: BiCGSTAB on a system already in device memory.
+    rowptr, col, val, b, diag (0: none) and x are device addresses on
+    `device`; x holds the guess and receives the solution.  The library
+    allocates nothing: its work space is a fixed static device array of
+    EP_QP_NOALLOC_MAX_ROWS rows.  Returns (error[ndim], iterations[ndim])."""
+    lib = L.load()
+    sys_ = L.EPQpSystem(n=int(n), ndim=int(ndim), nnz=int(nnz), rowptr=rowptr or None, col=col or None,
+                        val=val or None, diag=diag or None, b=b or None)
+    err = np.zeros(ndim, np.float32)
+    its = np.zeros(ndim, np.int32)
+    _err(lib, lib.ep_qp_solve_noalloc(C.byref(sys_), x or None, int(max_iterations), float(tolerance), int(device),
+                                      stream or None, err.ctypes.data, its.ctypes.data))
+    return err, its

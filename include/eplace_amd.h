@@ -338,6 +338,25 @@ int ep_qp_matrix(const ep_design *design, const float *node_xy, int32_t device, 
 int ep_qp_solve(const ep_qp_system *system, float *x, int32_t max_iterations, float tolerance, int32_t device,
                 float *error, int32_t *iterations);
 
+/* No counterpart in the reference.  ep_qp_solve for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): every array of `system` and `x` is a
+ * device pointer on `device` (x holds the guess and receives the solution),
+ * and the solver's work vectors, partial sums and scalars live in one
+ * library-owned static device array per device, sized for
+ * EP_QP_NOALLOC_MAX_ROWS rows (ndim * n; about 24 MiB, held until the
+ * process ends).  Larger systems are rejected with EP_ERR_ARG before any
+ * device work; a CSR whose rowptr or column indices are out of range is
+ * rejected by a checking kernel before the solve reads through it.  Kernels
+ * run on `stream` (a hipStream_t; NULL: the null stream) and the call
+ * returns when the solve has finished.  All calls on one device share the
+ * work space, which keeps the last call's contents: calls that overlap in
+ * time on one device race.  Results are those of ep_qp_solve on the same
+ * system.  error / iterations: [ndim] host arrays or NULL. */
+#define EP_QP_NOALLOC_MAX_ROWS (1 << 20)
+int ep_qp_solve_noalloc(const ep_qp_system *system, float *x, int32_t max_iterations, float tolerance,
+                        int32_t device, void *stream, float *error, int32_t *iterations);
+
 /* AbacusLegalizer::legalization (Legalization/legalizer.cpp:58-177) on the
  * intervals of PlaceDB::removeBlockedSite (placedb.cpp:629-849).  node_xy:
  * [num_nodes][2] centres (NULL: the design's); out_xy receives the legal
