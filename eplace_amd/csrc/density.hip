@@ -422,7 +422,8 @@ __global__ void __launch_bounds__(kBlock) gather_big_kernel(const int32_t *__res
 // combine (eplace.cpp:694-737) and, optionally, the Lipschitz partial sums
 // |g - g_prev|^2 and |y - y_prev|^2 over the stage's objects
 // (optutils.hpp:19-35) with, for the step controller, |grad WL|^2 and
-// |grad D|^2 over the stage.  Blocks walk the objects XCD-contiguously: with
+// |grad D|^2, and for the Barzilai-Borwein step (y - y_prev).(g - g_prev)
+// (nesterov.hpp:245-268) over the stage.  Blocks walk the objects XCD-contiguously: with
 // nets stored by degree class and then by their lowest node, the pin
 // gradients a range of nodes reads lie in one narrow band per class.
 __global__ void __launch_bounds__(kBlock) gather_combine_kernel(
@@ -435,7 +436,7 @@ __global__ void __launch_bounds__(kBlock) gather_combine_kernel(
     const DevScalars *sc) {
   if (sc->done) return;
   int o = xcd_block() * blockDim.x + threadIdx.x;
-  double ln = 0, ld = 0, nw = 0, nd = 0;
+  double ln = 0, ld = 0, nw = 0, nd = 0, sy = 0;
   if (o < O) {
     uint8_t k = kind[o];
     const float2 d = gd[o];  // from the tile / big-object gather
@@ -477,18 +478,18 @@ __global__ void __launch_bounds__(kBlock) gather_combine_kernel(
       ld = dxx * dxx + dxy * dxy;
       nw = (double)wl.x * wl.x + (double)wl.y * wl.y;
       nd = (double)d.x * d.x + (double)d.y * d.y;
+      sy = dxx * dgx + dxy * dgy;
     }
   }
   if (lipschitz) {
-    ln = block_sum(ln);
-    ld = block_sum(ld);
-    nw = block_sum(nw);
-    nd = block_sum(nd);
+    double v[5] = {ln, ld, nw, nd, sy};
+    block_sum_n(v);
     if (threadIdx.x == 0) {
-      partials[kPartialLip + 2 * blockIdx.x] = ln;
-      partials[kPartialLip + 2 * blockIdx.x + 1] = ld;
-      partials[kPartialNorm + 2 * blockIdx.x] = nw;
-      partials[kPartialNorm + 2 * blockIdx.x + 1] = nd;
+      partials[kPartialLip + 2 * blockIdx.x] = v[0];
+      partials[kPartialLip + 2 * blockIdx.x + 1] = v[1];
+      partials[kPartialNorm + 2 * blockIdx.x] = v[2];
+      partials[kPartialNorm + 2 * blockIdx.x + 1] = v[3];
+      partials[kPartialSY + 2 * blockIdx.x] = v[4];
     }
   }
 }

@@ -703,7 +703,8 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
   if (o.sync_every <= 0) o.sync_every = 16;
   if (o.max_backtracks <= 0) o.max_backtracks = 30;
   if (o.log_capacity < 0 || (o.log_capacity > 0 && !o.log)) return c.err = "bad log buffer", EP_ERR_ARG;
-  if (o.method < 0 || o.method > 2) return c.err = "method must be 0 (vanilla), 1 (backtracking) or 2 (learned)", EP_ERR_ARG;
+  if (o.method < 0 || o.method > 3)
+    return c.err = "method must be 0 (vanilla), 1 (backtracking), 2 (learned) or 3 (Barzilai-Borwein)", EP_ERR_ARG;
   const CtrlDev *ctrl = ctrl_of(o.controller);
   if (o.method == 2 && !ctrl) return c.err = "method 2 needs a controller", EP_ERR_ARG;
   if (ctrl && ctrl_device(ctrl) != c.device) return c.err = "controller is on another device", EP_ERR_ARG;
@@ -757,13 +758,14 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
     h.lambda_before = h.lambda;
     h.lam_mult = 1;
     h.backtracks_mark = h.trial = h.fallbacks = h.ctrl_active = 0;
+    h.bb_short = 0;
   });
   if (rc) return rc;
   launch_positions_to_ref(c, c.cur_main);
   // stop check before the first step uses tau of the last evaluation
   launch_done_check_ex(c, 0, nullptr, 0, 0);
   int k = 0;
-  const bool bk = o.method >= 1, learned = o.method == 2;
+  const bool bk = o.method == 1 || o.method == 2, learned = o.method == 2, bb = o.method == 3;
   const bool features = learned || dlog;  // controller inputs are logged for training
   while (true) {
     if (!bk) {
@@ -773,7 +775,7 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
         // these positions (only lambda changed since), later ones recompute
         if (k == 0 && c.eval_valid) launch_gather_combine(c, false, true, nullptr, c.last_ref, c.last_grad);
         else evaluate(c, k > 0, true, nullptr, c.last_ref, c.last_grad, false);
-        launch_step_scalar(c, 0);
+        launch_step_scalar(c, bb ? 1 : 0);
         launch_nesterov_move_ex(c, nullptr, c.gtot, c.cur_main, c.cur_main, nullptr, c.last_ref, c.last_grad);
         launch_done_check_ex(c, k + 1, dlog, cap, 1);
       }
@@ -788,7 +790,7 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
       } else {
         launch_gather_combine(c, false, true, nullptr, c.last_ref, c.last_grad, c.cur_ref, c.cur_grad);
       }
-      launch_step_scalar(c, 1);
+      launch_step_scalar(c, 0);
       if (features) launch_ctrl_step(c, ctrl, learned ? 1 : 0);
       int tries = 0;
       bool stopped = false;
@@ -874,6 +876,7 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
     out->final_overflow = final_tau;
     out->seconds = std::chrono::duration<double>(t1 - t0).count();
     out->controller_fallbacks = h.fallbacks;
+    out->bb_short_steps = h.bb_short;
   }
   return EP_OK;
 }

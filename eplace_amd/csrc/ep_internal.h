@@ -107,6 +107,8 @@ struct DevScalars {
   double step_prop, step_used, lambda_before, lam_mult;
   float feat[EP_CTRL_FEATURES];
   int backtracks_mark, trial, fallbacks, ctrl_active;
+  int bb_short;                       // iterations that took the short Barzilai-Borwein step
+  int pad1;
 };
 
 // Slots of the partial-sum buffer (doubles).
@@ -118,7 +120,8 @@ constexpr int kPartialLambda = 5 * kMaxPartials;  // |gwl|, |gd| pairs
 constexpr int kPartialRho = 7 * kMaxPartials;      // sum of rho per block
 constexpr int kPartialNorm = 8 * kMaxPartials;     // |gwl|^2, |gd|^2 pairs
 constexpr int kPartialRhoMax = 10 * kMaxPartials;  // max of rho per block
-constexpr int kPartialTotal = 11 * kMaxPartials;
+constexpr int kPartialSY = 11 * kMaxPartials;      // (y - y_prev).(g - g_prev), stride 2
+constexpr int kPartialTotal = 13 * kMaxPartials;
 
 struct Poisson;  // poisson.hip
 
@@ -283,7 +286,8 @@ void poisson_destroy(Ctx &c);
 void launch_poisson(Ctx &c, bool with_phi);
 // ---- nesterov.hip ----
 void launch_positions_to_ref(Ctx &c, float2 *dst);
-void launch_step_scalar(Ctx &c, int bktrk_first);
+// bb: the Barzilai-Borwein step (nesterov.hpp:231-302) instead of 1/L
+void launch_step_scalar(Ctx &c, int bb);
 void launch_accept(Ctx &c);
 void launch_lambda_init(Ctx &c);
 void launch_set_positions(Ctx &c, const float2 *xy_stage, const int32_t *stage_idx, int64_t n);

@@ -41,13 +41,19 @@ __global__ void copy_float2_kernel(int64_t n, float2 *dst, const float2 *src) {
 // (nesterov.hpp:375-415).  Iteration 0 uses step 1.  A non-finite 1/L
 // (identical consecutive positions, 0/0) keeps the previous step and is
 // counted in bad_numeric.
-__global__ void __launch_bounds__(kFinish) step_scalar_kernel(int nblocks, double *partials, DevScalars *sc) {
+// Step of an iteration: 1/L from the Lipschitz sums of the combine
+// (optutils.hpp:19-35; 1 at iteration 0), or with bb the Barzilai-Borwein
+// step of opt_step_bb (nesterov.hpp:245-281): the short step s.y / y.y when
+// positive, else min(|s| / |y|, 1/L), s = y_k - y_{k-1}, y = g_k - g_{k-1}.
+// Also the Nesterov coefficient a_{k+1} and the controller's gradient norms.
+__global__ void __launch_bounds__(kFinish) step_scalar_kernel(int nblocks, double *partials, DevScalars *sc,
+                                                              int bb) {
   if (sc->done) return;
-  const double *const src[4] = {partials + kPartialLip, partials + kPartialLip + 1, partials + kPartialNorm,
-                                partials + kPartialNorm + 1};
-  double sum[4];
+  const double *const src[5] = {partials + kPartialLip, partials + kPartialLip + 1, partials + kPartialNorm,
+                                partials + kPartialNorm + 1, partials + kPartialSY};
+  double sum[5];
   block_sum_partials_strided_n(src, nblocks, 2, sum);
-  const double num = sum[0], den = sum[1], nw = sum[2], nd = sum[3];
+  const double num = sum[0], den = sum[1], nw = sum[2], nd = sum[3], sy = sum[4];
   if (threadIdx.x == 0) {
     sc->gwl_norm2 = nw;
     sc->gd_norm2 = nd;
@@ -55,6 +61,15 @@ __global__ void __launch_bounds__(kFinish) step_scalar_kernel(int nblocks, doubl
     if (sc->iter > 0) {
       double L = sqrt(num) / sqrt(den);
       step = 1.0 / L;
+      if (bb) {
+        const double short_step = sy / num;
+        if (short_step > 0.0) {
+          step = short_step;
+          sc->bb_short += 1;
+        } else {
+          step = fmin(sqrt(den) / sqrt(num), step);
+        }
+      }
       if (!isfinite(step) || step <= 0.0) {
         sc->bad_numeric += 1;
         step = sc->step;
@@ -178,9 +193,9 @@ void launch_copy_float2(Ctx &c, float2 *dst, const float2 *src, int64_t n) {
   hipLaunchKernelGGL(copy_float2_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, c.stream, n, dst, src);
 }
 
-void launch_step_scalar(Ctx &c, int) {
+void launch_step_scalar(Ctx &c, int bb) {
   hipLaunchKernelGGL(step_scalar_kernel, dim3(1), dim3(kFinish), 0, c.stream, object_blocks(c.O), c.partials,
-                     c.sc);
+                     c.sc, bb);
 }
 
 void launch_accept(Ctx &c) {

@@ -2,7 +2,7 @@
 // (main/ePlace_main.cpp:35-223) over libeplace_amd.so.
 //
 //   ep_place -aux <design.aux> [-targetDensity 1.0] [-targetOverflow 0.10]
-//            [-loadpl <file.pl>] [-noQP] [-IPiteCount <n>] [-bktrk] [-LSE]
+//            [-loadpl <file.pl>] [-noQP] [-IPiteCount <n>] [-bktrk] [-bb] [-LSE]
 //            [-bins <n>] [-nomGP] [-internalLegal] [-outputPath <dir>]
 //            [-dumpDesign <file>]
 //
@@ -36,14 +36,16 @@ namespace {
 struct Args {
   std::string aux, loadpl, out = ".", dump;
   float td = 1.0f, tof = 0.10f;
-  bool bktrk = false, lse = false, no_mgp = false, legal = false, no_qp = false, no_mlg = false, no_cgp = false;
+  bool bktrk = false, bb = false, lse = false, no_mgp = false, legal = false, no_qp = false, no_mlg = false, no_cgp = false;
   int bins = 0, qp_iters = 20;
+  // -bb wins over -bktrk, as in EplaceNesterovOpt::opt_step (nesterov.hpp:157-165)
+  int method() const { return bb ? STEP_BB : bktrk ? STEP_BKTRK : STEP_VANILLA; }
 };
 
 int usage(const char *prog) {
   fprintf(stderr,
           "usage: %s -aux <design.aux> [-targetDensity d] [-targetOverflow t] [-loadpl f.pl]\n"
-          "          [-noQP] [-IPiteCount n] [-bktrk] [-LSE] [-bins n] [-nomGP] [-nomLG] [-nocGP]\n"
+          "          [-noQP] [-IPiteCount n] [-bktrk] [-bb] [-LSE] [-bins n] [-nomGP] [-nomLG] [-nocGP]\n"
           "          [-internalLegal]\n"
           "          [-outputPath dir] [-dumpDesign file]\n",
           prog);
@@ -78,6 +80,8 @@ bool parse(int argc, char **argv, Args &a) {
       a.bins = atoi(v.c_str());
     } else if (k == "-bktrk") {
       a.bktrk = true;
+    } else if (k == "-bb") {
+      a.bb = true;
     } else if (k == "-LSE") {
       a.lse = true;
     } else if (k == "-nomGP") {
@@ -170,7 +174,7 @@ int main(int argc, char **argv) {
     printf("bins %dx%d, %lld fillers, target density %.3f\n", info.bins_x, info.bins_y, (long long)info.num_fillers, a.td);
 
     if (!a.no_mgp) {
-      EplaceNesterovOpt<VECTOR_3D> opt(&placer, a.bktrk, a.tof);
+      EplaceNesterovOpt<VECTOR_3D> opt(&placer, a.method(), a.tof);
       opt.opt();
       const ep_run_stats &s = opt.last();
       printf("mGP finished: %d iterations, %d evaluations, overflow %.4f\n", s.iterations, s.evaluations,
@@ -190,11 +194,11 @@ int main(int argc, char **argv) {
              ms.outer_iterations, (long long)ms.perturbations, ms.final_overlap, ms.final_hpwl, ms.seconds);
       if (!a.no_cgp) {
         placer.switch2FillerOnly();
-        EplaceNesterovOpt<VECTOR_3D> fo(&placer, a.bktrk, a.tof);
+        EplaceNesterovOpt<VECTOR_3D> fo(&placer, a.method(), a.tof);
         fo.opt();
         printf("filler placement finished: %d iterations\n", fo.last().iterations);
         placer.switch2cGP();
-        EplaceNesterovOpt<VECTOR_3D> cg(&placer, a.bktrk, a.tof);
+        EplaceNesterovOpt<VECTOR_3D> cg(&placer, a.method(), a.tof);
         cg.opt();
         const ep_run_stats &s = cg.last();
         printf("cGP finished: %d iterations, overflow %.4f\n", s.iterations, s.final_overflow);

@@ -191,15 +191,20 @@ class FirstOrderOptimizer {
   virtual void opt() = 0;
 };
 
-// Nesterov with vanilla (default) or backtracking (-bktrk) step length.
+// Step-length variants of EplaceNesterovOpt::opt_step (nesterov.hpp:152-165):
+// vanilla (default), backtracking (-bktrk), Barzilai-Borwein (-bb, which
+// the reference checks first).  The values are ep_opt_params.method.
+enum StepMethod { STEP_VANILLA = 0, STEP_BKTRK = 1, STEP_BB = 3 };
+
+// Nesterov with the chosen step length.
 template <typename T>
 class EplaceNesterovOpt : public FirstOrderOptimizer<T> {
  public:
-  explicit EplaceNesterovOpt(EPlacer_2D *placer, bool bktrk = false, float targetOverflow = 0.10f)
-      : placer_(placer), bktrk_(bktrk), targetOverflow_(targetOverflow) {}
+  explicit EplaceNesterovOpt(EPlacer_2D *placer, int method = STEP_VANILLA, float targetOverflow = 0.10f)
+      : placer_(placer), method_(method), targetOverflow_(targetOverflow) {}
   void opt() override {
     ep_opt_params p{};
-    p.method = bktrk_ ? 1 : 0;
+    p.method = method_;
     p.target_overflow = targetOverflow_;
     p.max_iterations = 1000;  // MAX_ITERATION (nesterov.hpp:36)
     p.max_backtracks = 30;
@@ -214,7 +219,7 @@ class EplaceNesterovOpt : public FirstOrderOptimizer<T> {
 
  private:
   EPlacer_2D *placer_;
-  bool bktrk_;
+  int method_;
   float targetOverflow_;
   ep_run_stats last_{};
 };

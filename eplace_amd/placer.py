@@ -223,7 +223,7 @@ class EplaceNesterovOpt:
     whose first trial step and lambda multiplier come from `controller`).
     """
 
-    METHODS = {"vanilla": 0, "bktrk": 1, "learned": 2}
+    METHODS = {"vanilla": 0, "bktrk": 1, "learned": 2, "bb": 3}
 
     def __init__(self, placer: EPlacer2D, method: str = "vanilla", target_overflow: float = 0.10,
                  max_iterations: int = 1000, max_backtracks: int = 30, sync_every: int = 16, controller=None,

@@ -114,7 +114,9 @@ typedef struct ep_opt_params {
                                a factor on the penalty multiplier come from
                                `controller`; a first trial that fails the
                                backtracking acceptance test continues as plain
-                               backtracking */
+                               backtracking; 3 Barzilai-Borwein (-bb): one move per
+                               iteration with the short BB step s.y/y.y when positive,
+                               else min(|s|/|y|, 1/L) (nesterov.hpp:231-302) */
   float target_overflow;    /* mGP stop threshold (default 0.10) */
   int32_t max_iterations;   /* MAX_ITERATION (1000); stop when iter > this */
   int32_t max_backtracks;   /* cap on step retries per iteration (reference: none) */
@@ -137,7 +139,7 @@ typedef struct ep_run_stats {
   double final_hpwl;        /* HPWL after wrap_up (main solution written back) */
   double seconds;           /* wall time of the loop */
   int32_t controller_fallbacks; /* method 2: first trials that failed the acceptance test */
-  int32_t pad;
+  int32_t bb_short_steps;   /* method 3: iterations that took the short BB step */
 } ep_run_stats;
 
 /* Which device array ep_read/ep_write address (parity and debugging). */

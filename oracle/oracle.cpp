@@ -258,6 +258,7 @@ struct Placer : PlacerBase {
   // backtracking acceptance factor (BKTRK_EPS, nesterov.hpp:37); settable only
   // so that tests can plant a wrong value and see the trajectory check fail
   double bktrk_eps = 0.95;
+  int bb_short_used = 0;  // the last BB step was the short step
 
   // ---------------------------------------------------------------------
   void load(const orc_design *d, float target_density) {
@@ -1033,6 +1034,27 @@ struct Placer : PlacerBase {
     return std::sqrt(num) / std::sqrt(den);
   }
 
+  // Barzilai-Borwein step of opt_step_bb (nesterov.hpp:245-281): with
+  // s = y_k - y_{k-1} and y = g_k - g_{k-1} over the stage vector (z = 0),
+  // the short step s.y / y.y when positive, else min(|s| / |y|, 1/L).  The
+  // reference forms the norms and the dot product with Eigen's float
+  // reductions; here they are summed in order in T.
+  static T bb_step(const std::vector<T> &xt, const std::vector<T> &xt1, const std::vector<T> &gt,
+                   const std::vector<T> &gt1, T lip_step, int *used_short) {
+    T ss = 0, yy = 0, sy = 0;
+    for (size_t k = 0; k < xt.size(); ++k) {
+      T sk = xt[k] - xt1[k], yk = gt[k] - gt1[k];
+      ss += sk * sk;
+      yy += yk * yk;
+      sy += sk * yk;
+    }
+    T s_norm = std::sqrt(ss), y_norm = std::sqrt(yy);
+    T short_step = sy / (y_norm * y_norm);
+    *used_short = short_step > 0;
+    if (short_step > 0) return short_step;
+    return std::min(s_norm / y_norm, lip_step);
+  }
+
   // ---- EplaceNesterovOpt (nesterov.hpp:86-415) ----
   void opt_init() {
     iter = 0;
@@ -1060,12 +1082,17 @@ struct Placer : PlacerBase {
       nref[k] = np + (np - cur_main[k]) * coef;
     }
   }
-  // One opt_step (nesterov.hpp:151-200) with vanilla (375-415) or bktrk (310-369).
+  // One opt_step (nesterov.hpp:151-200) with vanilla (375-415), bktrk
+  // (310-369) or bb (231-302): method 0, 1 or 2.
   void opt_step(int bktrk, bool fixed_density, int shift) {
     get_position(cur_ref);
     total_gradient_update(fixed_density, shift);
     get_gradient(cur_grad);
     T step = iter == 0 ? (T)1 : (T)1 / lipschitz(cur_ref, last_ref, cur_grad, last_grad);
+    if (bktrk == 2) {
+      if (iter > 0) step = bb_step(cur_ref, last_ref, cur_grad, last_grad, step, &bb_short_used);
+      bktrk = 0;  // one move per iteration, as vanilla
+    }
     T a_new = ((T)1 + std::sqrt((T)4 * (ns_a * ns_a) + (T)1)) / (T)2;
     std::vector<T> nmain, nref, ngrad;
     if (!bktrk) {

@@ -1,7 +1,7 @@
 """Generate golden fixtures from the reference placer (run in the container
 that has the reference tree; the fixtures are committed, the reference is not).
 
-    python tests/golden/gen_golden.py [gp] [poisson] [legal] [qp] [mlg] [flow]   (default: all)
+    python tests/golden/gen_golden.py [gp] [bb] [poisson] [legal] [qp] [mlg] [flow]   (default: all)
 
 Builds oracle/_ref/ref_harness from the reference's own sources
 (oracle/Makefile.ref), runs it on the reference's smoke design and on seeded
@@ -66,6 +66,8 @@ def main(parts):
     tmp = tempfile.mkdtemp(prefix="golden_")
     if "gp" in parts:
         gp_fixtures(tmp)
+    if "bb" in parts:
+        bb_fixtures(tmp)
     if "poisson" in parts:
         poisson_fixtures(tmp)
     if "legal" in parts:
@@ -102,6 +104,19 @@ def gp_fixtures(tmp):
         run(["gp", "-aux", aux, "-targetDensity", str(td), "-targetOverflow", "0.1", "-noQP",
              "-loadpl", aux[:-4] + ".pl", "-dumpEvery", "1000"], tmp + "/out_" + name)
         pack(name, tmp + "/out_" + name, {"target_density": td})
+
+
+def bb_fixtures(tmp):
+    # the Barzilai-Borwein variant (-bb, nesterov.hpp:231-302): the smoke flow
+    # after QP, and the 2000-cell synthetic design from its .pl
+    smoke = os.path.join(REF, "examples", "smoke", "smoke.aux")
+    run(["gp", "-aux", smoke, "-targetDensity", "0.8", "-targetOverflow", "0.1", "-bb"], tmp + "/sbb")
+    pack("smoke_bb", tmp + "/sbb")
+    d = synthetic(num_cells=2000, num_nets=2100, seed=12, name="synth_c2")
+    aux = write_bookshelf(d, os.path.join(tmp, "synth_c2_bb"))
+    run(["gp", "-aux", aux, "-targetDensity", "1.0", "-targetOverflow", "0.1", "-noQP", "-loadpl",
+         aux[:-4] + ".pl", "-bb"], tmp + "/out_synth_c2_bb")
+    pack("synth_c2_bb", tmp + "/out_synth_c2_bb", {"target_density": 1.0})
 
 
 def poisson_fixtures(tmp):

@@ -84,7 +84,9 @@ def test_first_evaluation_vs_reference(name):
 @pytest.mark.parametrize("name,method,tol", [("smoke_default", "vanilla", 0.015),
                                              ("smoke_bktrk", "bktrk", 0.015),
                                              ("synth_c2", "vanilla", 0.005),
-                                             ("synth_mixed", "vanilla", 0.005)])
+                                             ("synth_mixed", "vanilla", 0.005),
+                                             ("smoke_bb", "bb", 0.015),
+                                             ("synth_c2_bb", "bb", 0.005)])
 def test_global_placement_vs_reference(name, method, tol):
     say(f"[golden-gp] {name} {method}")
     g = Golden(name)

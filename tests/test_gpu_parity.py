@@ -313,7 +313,7 @@ def test_lambda0_and_penalty_update():
 # it; on the 100-cell design the reference-float and FP64 oracle runs of the
 # same algorithm already differ by 1.2%, so 2% there.
 @pytest.mark.parametrize("case,tol", [("c2_small_64", 0.005), ("c1_16", 0.02)])
-@pytest.mark.parametrize("method", ["vanilla", "bktrk"])
+@pytest.mark.parametrize("method", ["vanilla", "bktrk", "bb"])
 def test_nesterov_run_matches_oracle(method, case, tol):
     say(f"[nesterov] method={method} case={case}")
     d, gp, orc = make(case)
@@ -324,7 +324,7 @@ def test_nesterov_run_matches_oracle(method, case, tol):
     orc.run("opt_init")
     k = 0
     while not orc.stop_condition(0.1):
-        orc.run("opt_step", 1 if method == "bktrk" else 0, shift)
+        orc.run("opt_step", {"vanilla": 0, "bktrk": 1, "bb": 2}[method], shift)
         k += 1
     orc.run("wrap_up")
     r = EplaceNesterovOpt(gp, method=method).opt()
@@ -336,7 +336,7 @@ def test_nesterov_run_matches_oracle(method, case, tol):
 def oracle_rows(orc, shift, method, K):
     rows = []
     for _ in range(K):
-        orc.run("opt_step", 1 if method == "bktrk" else 0, shift)
+        orc.run("opt_step", {"vanilla": 0, "bktrk": 1, "bb": 2}[method], shift)
         i = orc.info()
         rows.append(dict(tau=i["overflow"], lam=i["lambda"], hpwl=i["last_hpwl"], step=i["last_step"],
                          bk=i["bktrk_count"]))
@@ -352,15 +352,16 @@ TRAJ_TOL = dict(tau=1e-6, lam=1e-6, hpwl=1e-6, step=5e-6)
 
 
 @pytest.mark.parametrize("case", ["c1_16", "c2_small_64"])
-@pytest.mark.parametrize("method", ["vanilla", "bktrk"])
+@pytest.mark.parametrize("method", ["vanilla", "bktrk", "bb"])
 def test_trajectory_first_iterations(case, method):
     """Row-by-row agreement with the oracle for the first 12 iterations
     (before the runs separate: the dynamics are chaotic): overflow, lambda,
     HPWL and the Lipschitz step to about six digits, and under backtracking
     the same rejected trials at the same iterations (nesterov.hpp:310-369,
-    375-415).  Vanilla rows log lambda and HPWL one iteration late (the
-    penalty update at y_{k+1} runs inside the next evaluation), so those two
-    are compared with the oracle's previous row."""
+    375-415) and the Barzilai-Borwein step (231-302).  Vanilla and BB rows
+    log lambda and HPWL one iteration late (the penalty update at y_{k+1}
+    runs inside the next evaluation), so those two are compared with the
+    oracle's previous row."""
     say(f"[trajectory] case={case} method={method}")
     d, gp, orc = make(case)
     gp.initialization()
@@ -377,7 +378,7 @@ def test_trajectory_first_iterations(case, method):
     worst = {k: 0.0 for k in TRAJ_TOL}
     for k in range(K):
         gl, o = log[k], dict(otraj[k])
-        if method == "vanilla":
+        if method in ("vanilla", "bb"):
             if k == 0:
                 continue
             o["lam"], o["hpwl"] = otraj[k - 1]["lam"], otraj[k - 1]["hpwl"]

@@ -95,7 +95,8 @@ def test_first_iterations(name):
 # oracle's reference-float / FP64 / fixed-point variants of the -bktrk run end
 # between 240.8 and 243.0 (reference 240.68), so 1.5% there.
 @pytest.mark.parametrize("name,bktrk,tol", [("smoke_default", 0, 0.015), ("smoke_bktrk", 1, 0.015),
-                                            ("synth_c2", 0, 0.005), ("synth_mixed", 0, 0.005)])
+                                            ("synth_c2", 0, 0.005), ("synth_mixed", 0, 0.005),
+                                            ("smoke_bb", 2, 0.015), ("synth_c2_bb", 2, 0.005)])
 def test_full_run_matches_reference(name, bktrk, tol):
     g = Golden(name)
     d, o = oracle_at_init(g)
@@ -174,3 +175,27 @@ def test_bktrk_trajectory_check_catches_wrong_acceptance():
     T = g["trajectory"]
     rows = bktrk_rows(o, 9)
     assert any(np.float32(r[0]) != np.float32(T[k][1]) for k, r in enumerate(rows))
+
+
+@pytest.mark.parametrize("name,exact_rows", [("smoke_bb", 9), ("synth_c2_bb", 2)])
+def test_bb_trajectory_per_iteration(name, exact_rows):
+    """The reference's -bb run (nesterov.hpp:231-302) row by row: overflow,
+    HPWL and 1/gamma equal the reference's float values in the first rows;
+    the reference's Eigen float reductions of |s|, |y| and s.y, summed in
+    another order here, and the FFT rounding then move later rows by ~1e-6."""
+    g = Golden(name)
+    d, o = oracle_at_init(g)
+    o.run("total_gradient_update", 0, -1)
+    o.run("penalty_init")
+    o.run("opt_init")
+    T = g["trajectory"]
+    for k in range(12):
+        o.run("opt_step", 2, -1)
+        i = o.info()
+        t = T[k]
+        if k < exact_rows:
+            assert np.float32(i["overflow"]) == np.float32(t[1]), k
+            assert np.float32(i["last_hpwl"]) == np.float32(t[3]), k
+            assert np.float32(i["invgx"]) == np.float32(t[4]), k
+        for a, b in ((i["overflow"], t[1]), (i["last_hpwl"], t[3]), (i["invgx"], t[4]), (i["lambda"], t[2])):
+            assert abs(a - b) <= 1e-5 * abs(b), (k, a, b)
