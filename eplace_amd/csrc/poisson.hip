@@ -97,6 +97,8 @@ struct Poisson {
   float *spec = nullptr;  // [2hxp x 2hyp]
   float *PQt = nullptr;   // [2 nyp x 2hxp]: Pt rows 0..nyp, Qt rows nyp..2nyp
   float *wmax = nullptr;  // mode 2: per-wave max |output| of stages 1-3 [3][kMaxWaveMax]
+  void *bsplit = nullptr; // split modes: the current stage's data operand as planes
+  float *bscale = nullptr; // mode 2: the scale split_kernel applied, per stage [4]
 };
 
 constexpr int kMaxWaveMax = 1 << 16;
@@ -122,13 +124,11 @@ struct PGemm {
   int epi;             // 1: divide by wx2p[row] + wy2p[col], zero the DC term
   const double *sub;   // B := B - *sub before folding (removes the mean of rho)
   const float *wx2p, *wy2p;
-  // mode 2: bound on |B| before folding: the maximum of in_max[0..in_n) or,
-  // if in_max_d is set, *in_max_d; out_max (optional) receives one max |C|
-  // per wave of the launch
-  const float *in_max;
-  int in_n;
-  const double *in_max_d;
-  float *out_max;
+  // split modes: B as planes written by split_kernel, rows of ldbp
+  const void *Bp[3][kMaxPlanes];
+  int ldbp;
+  const float *bscale;  // mode 2: the power of two split_kernel scaled B by
+  float *out_max;       // mode 2 (optional): one max |C| per wave of the launch
 };
 
 constexpr int PBM = 64, PBN = 64, PK = 32;
@@ -164,8 +164,7 @@ struct Stage32 {
   float a[8], b[8];
 };
 template <int MODE> struct StageS {
-  typename Split<MODE>::V a[Split<MODE>::NP];
-  float b[8];
+  typename Split<MODE>::V a[Split<MODE>::NP], b[Split<MODE>::NP];
 };
 struct Smem32 {
   float A[2][PBM][kLd32], B[2][PBN][kLd32];
@@ -212,16 +211,20 @@ __device__ __forceinline__ void load_stage(const PGemm &p, int z, const float *B
   load_b(p, B, sb, sub, col0, k0, st.b);
 }
 
+// Split modes: both operands arrive as planes (A stored split at set-up, B
+// split once per stage by split_kernel), one 16-byte load per plane.
 template <int MODE>
-__device__ __forceinline__ void load_stage(const PGemm &p, int z, const float *B, float sb, float sub, int row0,
-                                           int col0, int k0, StageS<MODE> &st) {
+__device__ __forceinline__ void load_stage(const PGemm &p, int z, const float *, float, float, int row0, int col0,
+                                           int k0, StageS<MODE> &st) {
   typedef typename Split<MODE>::T T;
   typedef typename Split<MODE>::V V;
   const int r = gtid() >> 2, k = k0 + 8 * (gtid() & 3);
-  const size_t off = (size_t)(row0 + r) * p.lda + k;
+  const size_t offa = (size_t)(row0 + r) * p.lda + k, offb = (size_t)(col0 + r) * p.ldbp + k;
 #pragma unroll
-  for (int q = 0; q < Split<MODE>::NP; ++q) st.a[q] = *reinterpret_cast<const V *>(static_cast<const T *>(p.Ap[z][q]) + off);
-  load_b(p, B, sb, sub, col0, k0, st.b);
+  for (int q = 0; q < Split<MODE>::NP; ++q) {
+    st.a[q] = *reinterpret_cast<const V *>(static_cast<const T *>(p.Ap[z][q]) + offa);
+    st.b[q] = *reinterpret_cast<const V *>(static_cast<const T *>(p.Bp[z][q]) + offb);
+  }
 }
 
 __device__ __forceinline__ void store_stage(Smem32 &s, int buf, const Stage32 &st, float) {
@@ -232,29 +235,15 @@ __device__ __forceinline__ void store_stage(Smem32 &s, int buf, const Stage32 &s
   *reinterpret_cast<float4 *>(&s.B[buf][r][k + 4]) = make_float4(st.b[4], st.b[5], st.b[6], st.b[7]);
 }
 
-// B is split here: plane q holds the rounding of what planes 0..q-1 left
-// (after the mode-2 scale, a power of two).
 template <int MODE>
-__device__ __forceinline__ void store_stage(SmemS<MODE> &s, int buf, const StageS<MODE> &st, float bscale) {
-  typedef typename Split<MODE>::T T;
+__device__ __forceinline__ void store_stage(SmemS<MODE> &s, int buf, const StageS<MODE> &st, float) {
   typedef typename Split<MODE>::V V;
-  constexpr int NP = Split<MODE>::NP;
   const int r = gtid() >> 2, k = swz16(r, gtid() & 3);
 #pragma unroll
-  for (int q = 0; q < NP; ++q) *reinterpret_cast<V *>(&s.A[buf][q][r][k]) = st.a[q];
-  V pl[NP];
-#pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    float rem = MODE == 2 ? st.b[i] * bscale : st.b[i];
-#pragma unroll
-    for (int q = 0; q < NP; ++q) {
-      const T x = (T)rem;
-      pl[q][i] = x;
-      rem -= (float)x;
-    }
+  for (int q = 0; q < Split<MODE>::NP; ++q) {
+    *reinterpret_cast<V *>(&s.A[buf][q][r][k]) = st.a[q];
+    *reinterpret_cast<V *>(&s.B[buf][q][r][k]) = st.b[q];
   }
-#pragma unroll
-  for (int q = 0; q < NP; ++q) *reinterpret_cast<V *>(&s.B[buf][q][r][k]) = pl[q];
 }
 
 // FP32: lane group g = lane / 16 takes k = 8 g + s in sub-step s for both
@@ -318,39 +307,82 @@ __device__ __forceinline__ void mma(const SmemS<MODE> &s, int buf, floatx4 (&acc
         for (int qa = ord; qa >= 0; --qa) acc[mi][ni] = mfma16(a[mi][qa], b[ni][ord - qa], acc[mi][ni]);
 }
 
+// Register stages per k-group: the global loads of this many rounds are in
+// flight while a round multiplies.
+#ifndef EP_PGEMM_DEPTH
+#define EP_PGEMM_DEPTH 2
+#endif
+constexpr int kDepth = EP_PGEMM_DEPTH;
+
 // acc += A[rows, k_begin..k_end) B[cols, k_begin..k_end)^T, the k steps
 // split between the block's k-groups (group g takes steps g, g + kGroups,
-// ...).  Each group has its own LDS double buffer and two register stages:
-// while round r multiplies from LDS, the global loads of rounds r + 1 and
-// r + 2 are in flight (round r + 1's go to LDS at the end of round r).
-// Every group passes the same number of block barriers (one per round),
-// also when it has no step left.
+// ...).  Each group has its own LDS double buffer and kDepth register
+// stages: while round r multiplies from LDS, the global loads of rounds
+// r + 1 .. r + kDepth are in flight (round r + 1's go to LDS at the end of
+// round r, and round r's stage then takes round r + kDepth's loads).  Every
+// group passes the same number of block barriers (one per round), also when
+// it has no step left.
 template <class ST, class SM>
 __device__ __forceinline__ void run_k(const PGemm &p, int z, const float *B, float sb, float sub, float bscale,
                                       int row0, int col0, int k_begin, int k_end, SM &s, floatx4 (&acc)[2][2]) {
-  const int grp = threadIdx.x / kGroupThreads;
-  const int steps = (k_end - k_begin + PK - 1) / PK;
-  const int rounds = (steps + kGroups - 1) / kGroups;
+  // K spans of a run are multiples of kGroups * PK (the host checks), so
+  // every group runs the same rounds and every test below depends only on
+  // kernel arguments: branches stay scalar, and the loads of the next
+  // rounds stay in flight across them.
+  const int grp = __builtin_amdgcn_readfirstlane(threadIdx.x / kGroupThreads);
+  const int rounds = (k_end - k_begin) / (PK * kGroups);
   auto kstep = [&](int r) { return k_begin + (r * kGroups + grp) * PK; };
-  auto have = [&](int r) { return kstep(r) < k_end; };
-  ST even, odd;  // register stages of the even and the odd rounds
-  if (have(0)) load_stage(p, z, B, sb, sub, row0, col0, kstep(0), even);
-  if (have(1)) load_stage(p, z, B, sb, sub, row0, col0, kstep(1), odd);
+  ST st[kDepth];  // st[r % kDepth] holds round r's loads until they go to LDS
+  if (rounds >= 2 * kDepth && rounds % kDepth == 0) {
+    // Branch-free pipeline: the wait before each LDS store then covers
+    // exactly the loads of the round it stores, with kDepth - 1 later
+    // rounds still in flight.
+#pragma unroll
+    for (int i = 0; i < kDepth; ++i) load_stage(p, z, B, sb, sub, row0, col0, kstep(i), st[i]);
+    __syncthreads();  // the previous pass may still read buffer 0
+    store_stage(s, 0, st[0], bscale);
+    __syncthreads();
+    int r0 = 0;
+    for (; r0 < rounds - kDepth; r0 += kDepth) {
+#pragma unroll
+      for (int j = 0; j < kDepth; ++j) {
+        const int r = r0 + j;
+        load_stage(p, z, B, sb, sub, row0, col0, kstep(r + kDepth), st[j]);
+        // keep the loads ahead of the multiply (the scheduler would sink
+        // them below the next LDS store, leaving one round to cover them)
+        __builtin_amdgcn_sched_barrier(0);
+        mma(s, r & 1, acc);
+        store_stage(s, (r + 1) & 1, st[(j + 1) % kDepth], bscale);
+        __syncthreads();
+      }
+    }
+    // the last kDepth rounds: nothing left to load
+#pragma unroll
+    for (int j = 0; j < kDepth; ++j) {
+      const int r = r0 + j;
+      mma(s, r & 1, acc);
+      if (j + 1 < kDepth) store_stage(s, (r + 1) & 1, st[(j + 1) % kDepth], bscale);
+      __syncthreads();
+    }
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < kDepth; ++i)
+    if (i < rounds) load_stage(p, z, B, sb, sub, row0, col0, kstep(i), st[i]);
   __syncthreads();  // the previous pass may still read buffer 0
-  if (have(0)) store_stage(s, 0, even, bscale);
+  if (rounds > 0) store_stage(s, 0, st[0], bscale);
   __syncthreads();
-  for (int r = 0; r < rounds; r += 2) {
-    // round r: buffer 0 holds step r, `odd` step r + 1, `even` is free
-    if (have(r + 2)) load_stage(p, z, B, sb, sub, row0, col0, kstep(r + 2), even);
-    if (have(r)) mma(s, 0, acc);
-    if (have(r + 1)) store_stage(s, 1, odd, bscale);
-    __syncthreads();
-    if (r + 1 >= rounds) break;
-    // round r + 1: buffer 1 holds step r + 1, `even` step r + 2, `odd` is free
-    if (have(r + 3)) load_stage(p, z, B, sb, sub, row0, col0, kstep(r + 3), odd);
-    if (have(r + 1)) mma(s, 1, acc);
-    if (have(r + 2)) store_stage(s, 0, even, bscale);
-    __syncthreads();
+  for (int r0 = 0; r0 < rounds; r0 += kDepth) {
+#pragma unroll
+    for (int j = 0; j < kDepth; ++j) {
+      const int r = r0 + j;
+      if (r >= rounds) break;
+      // LDS buffer r & 1 holds round r; its stage st[j] is free again
+      if (r + kDepth < rounds) load_stage(p, z, B, sb, sub, row0, col0, kstep(r + kDepth), st[j]);
+      mma(s, r & 1, acc);
+      if (r + 1 < rounds) store_stage(s, (r + 1) & 1, st[(j + 1) % kDepth], bscale);
+      __syncthreads();
+    }
   }
 }
 
@@ -402,28 +434,14 @@ __global__ void __launch_bounds__(kGroups * kGroupThreads) pgemm_kernel(PGemm p)
   float *__restrict__ C = p.C[z];
   const float sb = p.sb[z];
   const float sub = p.sub ? (float)*p.sub : 0.f;
-  const int grp = threadIdx.x / kGroupThreads;
+  const int grp = __builtin_amdgcn_readfirstlane(threadIdx.x / kGroupThreads);
   const int lane = threadIdx.x & 63, wave = gtid() >> 6;
   const int row0 = blockIdx.y * PBM, col0 = blockIdx.x * PBN;
   const int roww = row0 + (wave >> 1) * 32, colw = col0 + (wave & 1) * 32;
-  // mode 2: the data operand's scale from the bound the previous stage left
+  // mode 2: undo the operands' power-of-two scales in the epilogue
   float bscale = 1.f, out_scale = 1.f;
   if (MODE == 2) {
-    float m = 0.f;
-    if (p.in_max_d) {
-      m = (float)*p.in_max_d;
-    } else {
-      __shared__ float wm[kGroups * kGroupThreads / 64];
-      for (int i = threadIdx.x; i < p.in_n; i += blockDim.x) m = fmaxf(m, p.in_max[i]);
-      for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
-      if (lane == 0) wm[threadIdx.x >> 6] = m;
-      __syncthreads();
-      m = wm[0];
-#pragma unroll
-      for (int i = 1; i < kGroups * kGroupThreads / 64; ++i) m = fmaxf(m, wm[i]);
-    }
-    // folded B adds two values of at most m
-    bscale = f16_scale(m);
+    bscale = *p.bscale;
     out_scale = p.a_inv[z] / bscale;
   }
   floatx4 acc[2][2], odd[2][2];
@@ -486,10 +504,120 @@ __global__ void __launch_bounds__(kGroups * kGroupThreads) pgemm_kernel(PGemm p)
   }
 }
 
+// One stage's data operand split into planes for the split-mode GEMMs:
+// batch z, row n, columns k < K of
+//   v = (src[n][k] - sub) + sb_z (src[n][nfb-1-k] - sub) for k < hb, 0 beyond  (fold)
+//   v = src[n][k]                                                              (no fold)
+// times the mode-2 scale, stored as plane q at out + ((z NP + q) rows + n) K + k.
+// Each element is split once here instead of once per GEMM block that reads
+// it.  Mode 2 takes the scale from the bound on |src| (the largest of
+// in_max[0..in_n), or *in_max_d) and stores it for the GEMM's epilogue.
+struct SplitJob {
+  const float *src;
+  int ld, rows, K, nbatch;
+  float sb[2];
+  int fold, nfb, hb;
+  const double *sub;
+  const float *in_max;
+  int in_n;
+  const double *in_max_d;
+  float *scale_out;
+  void *out;
+};
+
+constexpr int kSplitThreads = 256;
+
+template <int MODE>
+__global__ void __launch_bounds__(kSplitThreads) split_kernel(SplitJob j) {
+  typedef typename Split<MODE>::T T;
+  typedef typename Split<MODE>::V V;
+  constexpr int NP = Split<MODE>::NP;
+  float scale = 1.f;
+  if (MODE == 2) {
+    float m = 0.f;
+    if (j.in_max_d) {
+      m = (float)*j.in_max_d;
+    } else {
+      __shared__ float wm[kSplitThreads / 64];
+      for (int i = threadIdx.x; i < j.in_n; i += blockDim.x) m = fmaxf(m, j.in_max[i]);
+      for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+      if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+      __syncthreads();
+      m = wm[0];
+#pragma unroll
+      for (int i = 1; i < kSplitThreads / 64; ++i) m = fmaxf(m, wm[i]);
+    }
+    scale = f16_scale(m);  // folded values add two of at most m
+    if (blockIdx.x == 0 && threadIdx.x == 0) *j.scale_out = scale;
+  }
+  const float sub = j.sub ? (float)*j.sub : 0.f;
+  const int per_row = j.K / 8;
+  const int64_t per_batch = (int64_t)j.rows * per_row, total = per_batch * j.nbatch;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int z = (int)(t / per_batch);
+    const int64_t rem = t - z * per_batch;
+    const int n = (int)(rem / per_row), k = 8 * (int)(rem - (int64_t)n * per_row);
+    const float *row = j.src + (size_t)n * j.ld;
+    float v[8];
+    if (j.fold) {
+      const float sb = j.sb[z];
+      if ((j.nfb & 3) == 0 && k + 8 <= j.hb) {
+        // k + i and its mirror nfb - 1 - k - i: two aligned float4 each
+        const float4 a0 = *reinterpret_cast<const float4 *>(row + k);
+        const float4 a1 = *reinterpret_cast<const float4 *>(row + k + 4);
+        const float4 m0 = *reinterpret_cast<const float4 *>(row + j.nfb - 8 - k);
+        const float4 m1 = *reinterpret_cast<const float4 *>(row + j.nfb - 4 - k);
+        const float a[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+        const float mv[8] = {m1.w, m1.z, m1.y, m1.x, m0.w, m0.z, m0.y, m0.x};
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = (a[i] - sub) + sb * (mv[i] - sub);
+      } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+          v[i] = k + i < j.hb ? (row[k + i] - sub) + sb * (row[j.nfb - 1 - k - i] - sub) : 0.f;
+      }
+    } else {
+      const float4 a0 = *reinterpret_cast<const float4 *>(row + k);
+      const float4 a1 = *reinterpret_cast<const float4 *>(row + k + 4);
+      v[0] = a0.x; v[1] = a0.y; v[2] = a0.z; v[3] = a0.w;
+      v[4] = a1.x; v[5] = a1.y; v[6] = a1.z; v[7] = a1.w;
+    }
+    V pl[NP];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      float r = MODE == 2 ? v[i] * scale : v[i];
+#pragma unroll
+      for (int q = 0; q < NP; ++q) {
+        const T x = (T)r;
+        pl[q][i] = x;
+        r -= (float)x;
+      }
+    }
+    T *out = static_cast<T *>(j.out);
+#pragma unroll
+    for (int q = 0; q < NP; ++q)
+      *reinterpret_cast<V *>(out + ((size_t)(z * NP + q) * j.rows + n) * j.K + k) = pl[q];
+  }
+}
+
+static void split(Ctx &c, const SplitJob &j) {
+  const int64_t chunks = (int64_t)j.nbatch * j.rows * (j.K / 8);
+  const int blocks = (int)std::min<int64_t>(256, (chunks + kSplitThreads - 1) / kSplitThreads);
+  switch (c.poisson->mode) {
+    case 1: hipLaunchKernelGGL(split_kernel<1>, dim3(blocks), dim3(kSplitThreads), 0, c.stream, j); break;
+    case 2: hipLaunchKernelGGL(split_kernel<2>, dim3(blocks), dim3(kSplitThreads), 0, c.stream, j); break;
+    default: hipLaunchKernelGGL(split_kernel<3>, dim3(blocks), dim3(kSplitThreads), 0, c.stream, j); break;
+  }
+}
+
 // Number of per-wave maxima a launch of this shape writes.
 static int wave_max_count(const PGemm &p, int batch) {
   return (p.N / PBN) * (p.M / PBM) * batch * (kGroupThreads / 64);
 }
+
+// Every K span is a multiple of the 64-element padding of the halves, so
+// run_k's rounds are whole for both k-groups.
+static_assert(64 % (PK * kGroups) == 0, "K spans must hold whole rounds");
 
 static void gemm(Ctx &c, const PGemm &p, int batch) {
   dim3 grid(p.N / PBN, p.M / PBM, batch);
@@ -606,9 +734,17 @@ bool poisson_create(Ctx &c, std::string &err) {
     return hipMemset(d, 0, n * sizeof(float)) == hipSuccess;
   };
   const int md = q->mode;
+  // planes of the largest stage operand: stage 1 (2 batches x X rows x Hyp),
+  // stage 2 (2 x 2Hyp x Hxp), stage 3 (2Hxp x 2Hyp), stage 4 (2Y x 2Hxp)
+  const size_t split_elems = std::max(std::max((size_t)2 * X * Hyp, (size_t)4 * Hyp * Hxp),
+                                      std::max((size_t)4 * Hxp * Hyp, (size_t)4 * Y * Hxp));
+  const int np = md == 3 ? 3 : 2;
   bool ok = upload(q->FYA, FYA, md) && upload(q->FXA, FXA, md) && upload(q->CYT, CYT, md) &&
             upload(q->SYT, SYT, md) && upload(q->CX2, CX2, md) && upload(q->SX2, SX2, md) && up(q->wx2p, wx2p) &&
             up(q->wy2p, wy2p) && (md != 2 || zeros(q->wmax, (size_t)3 * kMaxWaveMax)) &&
+            (md == 0 || (hipMalloc(&q->bsplit, split_elems * np * 2) == hipSuccess &&
+                         hipMemset(q->bsplit, 0, split_elems * np * 2) == hipSuccess)) &&
+            (md == 0 || zeros(q->bscale, 4)) &&
             zeros(q->T1t, (size_t)2 * Hyp * X) && zeros(q->spec, (size_t)2 * Hxp * 2 * Hyp) &&
             zeros(q->PQt, (size_t)2 * Y * 2 * Hxp) && zeros(c.rho, (size_t)X * Y) && zeros(c.phi, (size_t)X * Y) &&
             zeros(c.ex, (size_t)X * Y) && zeros(c.ey, (size_t)X * Y);
@@ -628,7 +764,8 @@ void poisson_destroy(Ctx &c) {
   if (!q) return;
   Mat *mats[] = {&q->FYA, &q->FXA, &q->CYT, &q->SYT, &q->CX2, &q->SX2};
   for (Mat *m : mats) release(*m);
-  float *ptrs[] = {q->wx2p, q->wy2p, q->T1t, q->spec, q->PQt, q->wmax};
+  float *ptrs[] = {q->wx2p, q->wy2p, q->T1t, q->spec, q->PQt, q->wmax, q->bscale};
+  if (q->bsplit) (void)hipFree(q->bsplit);
   for (float *p : ptrs)
     if (p) (void)hipFree(p);
   delete q;
@@ -649,11 +786,22 @@ static void set_a(PGemm &p, int z, const Mat &m, size_t off) {
 void launch_poisson(Ctx &c, bool with_phi) {
   Poisson *q = c.poisson;
   const int X = q->nxp, Y = q->nyp, Hxp = q->hxp, Hyp = q->hyp;
+  const bool split_mode = q->mode != 0;
+  const int np = q->mode == 3 ? 3 : 2;
+  // split modes: plane qq of batch z of a split of `rows` rows of K, from row r0
+  auto plane = [&](int z, int qq, int rows, int K, int r0) -> const void * {
+    return static_cast<const char *>(q->bsplit) + (((size_t)(z * np + qq) * rows + r0) * K) * 2;
+  };
+  auto set_b = [&](PGemm &p, int z, int bz, int rows, int K, int r0) {
+    for (int qq = 0; qq < np; ++qq) p.Bp[z][qq] = plane(bz, qq, rows, K, r0);
+    p.ldbp = K;
+  };
   PGemm p = {};
+  SplitJob j = {};
   // 1. T1t[v][x] = sum_y F_Y[v][y] (rho[x][y] - mean), even v on fold +,
   // odd v on fold -.  A constant only moves rho^[0][0], which the solve
   // discards; removing it keeps its float rounding out of the low
-  // frequencies.
+  // frequencies.  Mode 2 scales rho by its largest value.
   p.sub = &c.sc->rho_mean;
   set_a(p, 0, q->FYA, 0);
   set_a(p, 1, q->FYA, (size_t)Hyp * Hyp);
@@ -665,10 +813,17 @@ void launch_poisson(Ctx &c, bool with_phi) {
   p.lda = Hyp; p.ldb = Y; p.ldc = X;
   p.M = Hyp; p.N = X; p.K = Hyp;
   p.nfb = c.ny; p.hb = q->hy;
-  // mode 2 scales: stage 1 from the largest charge density, later stages
-  // from the per-wave maxima of the stage before
-  p.in_max_d = &c.sc->rho_max;
   p.out_max = q->wmax;
+  if (split_mode) {
+    j = SplitJob{};
+    j.src = c.rho; j.ld = Y; j.rows = X; j.K = Hyp; j.nbatch = 2;
+    j.sb[0] = 1.f; j.sb[1] = -1.f; j.fold = 1; j.nfb = c.ny; j.hb = q->hy;
+    j.sub = &c.sc->rho_mean; j.in_max_d = &c.sc->rho_max; j.scale_out = q->bscale; j.out = q->bsplit;
+    split(c, j);
+    set_b(p, 0, 0, X, Hyp, 0);
+    set_b(p, 1, 1, X, Hyp, 0);
+    p.bscale = q->bscale;
+  }
   int n_prev = wave_max_count(p, 2);
   gemm(c, p, 2);
   // 2. spec[u][v] = sum_x F_X[u][x] T1t[v][x] / (wx^2 + wy^2)
@@ -688,9 +843,17 @@ void launch_poisson(Ctx &c, bool with_phi) {
   p.epi = 1;
   p.wx2p = q->wx2p;
   p.wy2p = q->wy2p;
-  p.in_max = q->wmax;
-  p.in_n = n_prev;
   p.out_max = q->wmax + kMaxWaveMax;
+  if (split_mode) {
+    j = SplitJob{};
+    j.src = q->T1t; j.ld = X; j.rows = 2 * Hyp; j.K = Hxp; j.nbatch = 2;
+    j.sb[0] = 1.f; j.sb[1] = -1.f; j.fold = 1; j.nfb = c.nx; j.hb = q->hx;
+    j.in_max = q->wmax; j.in_n = n_prev; j.scale_out = q->bscale + 1; j.out = q->bsplit;
+    split(c, j);
+    set_b(p, 0, 0, 2 * Hyp, Hxp, 0);
+    set_b(p, 1, 1, 2 * Hyp, Hxp, 0);
+    p.bscale = q->bscale + 1;
+  }
   n_prev = wave_max_count(p, 2);
   gemm(c, p, 2);
   // 3. Pt[y][u] = sum_v C_Y[v][y] spec[u][v] (mirror: e - o),
@@ -706,9 +869,16 @@ void launch_poisson(Ctx &c, bool with_phi) {
   p.lda = 2 * Hyp; p.ldb = 2 * Hyp; p.ldc = 2 * Hxp;
   p.M = Hyp; p.N = 2 * Hxp; p.K = 2 * Hyp; p.khalf = Hyp;
   p.out_fold = 1; p.nfo = c.ny; p.ho = q->hy;
-  p.in_max = q->wmax + kMaxWaveMax;
-  p.in_n = n_prev;
   p.out_max = q->wmax + 2 * kMaxWaveMax;
+  if (split_mode) {
+    j = SplitJob{};
+    j.src = q->spec; j.ld = 2 * Hyp; j.rows = 2 * Hxp; j.K = 2 * Hyp; j.nbatch = 1;
+    j.in_max = q->wmax + kMaxWaveMax; j.in_n = n_prev; j.scale_out = q->bscale + 2; j.out = q->bsplit;
+    split(c, j);
+    set_b(p, 0, 0, 2 * Hxp, 2 * Hyp, 0);
+    set_b(p, 1, 0, 2 * Hxp, 2 * Hyp, 0);
+    p.bscale = q->bscale + 2;
+  }
   n_prev = wave_max_count(p, 2);
   gemm(c, p, 2);
   // 4. Ex[x][y] = sum_u wx S_X[u][x] Pt[y][u] (mirror: o - e),
@@ -729,8 +899,16 @@ void launch_poisson(Ctx &c, bool with_phi) {
   p.lda = 2 * Hxp; p.ldb = 2 * Hxp; p.ldc = Y;
   p.M = Hxp; p.N = Y; p.K = 2 * Hxp; p.khalf = Hxp;
   p.out_fold = 1; p.nfo = c.nx; p.ho = q->hx;
-  p.in_max = q->wmax + 2 * kMaxWaveMax;
-  p.in_n = n_prev;
+  if (split_mode) {
+    j = SplitJob{};
+    j.src = q->PQt; j.ld = 2 * Hxp; j.rows = 2 * Y; j.K = 2 * Hxp; j.nbatch = 1;
+    j.in_max = q->wmax + 2 * kMaxWaveMax; j.in_n = n_prev; j.scale_out = q->bscale + 3; j.out = q->bsplit;
+    split(c, j);
+    set_b(p, 0, 0, 2 * Y, 2 * Hxp, 0);
+    set_b(p, 1, 0, 2 * Y, 2 * Hxp, Y);
+    set_b(p, 2, 0, 2 * Y, 2 * Hxp, 0);
+    p.bscale = q->bscale + 3;
+  }
   gemm(c, p, with_phi ? 3 : 2);
 }
 

@@ -259,6 +259,9 @@ struct Placer : PlacerBase {
   // so that tests can plant a wrong value and see the trajectory check fail
   double bktrk_eps = 0.95;
   int bb_short_used = 0;  // the last BB step was the short step
+  // DELTA_HPWL_REF (eplace.h:11); settable only for experiments on how the
+  // absolute reference throttles lambda on large designs
+  double delta_hpwl_ref = 350000;
 
   // ---------------------------------------------------------------------
   void load(const orc_design *d, float target_density) {
@@ -976,14 +979,14 @@ struct Placer : PlacerBase {
       float mult;
       double delta = cur - last_hpwl;
       if (delta < 0.0) mult = (float)1.05;
-      else mult = (float)std::pow(1.05, (-(delta) / 350000 + 1.0));
+      else mult = (float)std::pow(1.05, (-(delta) / delta_hpwl_ref + 1.0));
       if (float_greater(mult, (float)1.05)) mult = (float)1.05;
       if (float_less(mult, (float)0.95)) mult = (float)0.95;
       lambda *= mult;
       last_hpwl = cur;
     } else {
       double delta = hp - last_hpwl;
-      double mult = delta < 0.0 ? 1.05 : std::pow(1.05, -delta / 350000 + 1.0);
+      double mult = delta < 0.0 ? 1.05 : std::pow(1.05, -delta / delta_hpwl_ref + 1.0);
       mult = std::min(1.05, std::max(0.95, mult));
       lambda = (T)((double)lambda * mult);
       last_hpwl = hp;
@@ -1218,6 +1221,7 @@ void orc_set_scalar(void *p, const char *name, double v) {
     else if (n == "poisson_fast") P->poisson_fast = (int)v;
     else if (n == "exp_mode") P->exp_mode = (int)v;
     else if (n == "bktrk_eps") P->bktrk_eps = v;
+    else if (n == "delta_hpwl_ref") P->delta_hpwl_ref = v;
   });
 }
 
