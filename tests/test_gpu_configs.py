@@ -10,15 +10,20 @@ through properties that do not need a reference run of the same size:
   reaches the stage targets and legal standard cells;
 * 2M std cells, 2048x2048 bins, split-FP16 transforms: the learned
   controller and plain backtracking both reach overflow 0.10, the learned
-  run in fewer gradient evaluations and within 1% of the plain HPWL.
-  (With bins 1.34 rows high the overflow of plain backtracking levels off
-  between 0.08 and 0.10 by 1000-3000 iterations, so 0.07 is not asked.)
+  run in fewer gradient evaluations and within 1% of the plain HPWL;
+* 2M std cells + 200 movable macros (total macro area capped at 45% of the
+  die), utilisation 0.8, 2048x2048 bins, target overflow 0.07: mGP -> mLG
+  -> FILLERONLY -> cGP with the reference's stop rules (nesterov.hpp:99-117)
+  and staging (ePlace_main.cpp:176-219).
 
-mGP on the macro-heavy 500K design (macros cover about 40% of the die)
-stops at its iteration cap, as the reference's loop does when the target
-is not reached; the test asks for the overflow to fall below 0.5.  (Two
-hundred macros of 3-12% of the die width would cover more than the whole
-die, so the 2M case is run with standard cells only.)  The annealing macro
+mGP on the macro-heavy designs stops at its iteration cap, as the
+reference's loop does when the target is not reached.  The cause is the
+reference's penalty rule: lambda grows by 1.05^(1 - dHPWL/350000)
+(eplace.cpp:846-883), an absolute HPWL reference, and on these designs HPWL
+moves by 1e7 or more per iteration, so lambda random-walks instead of
+growing.  The FP64 oracle started from the GPU's state on the 2M-cell,
+200-macro design follows the GPU's trajectory to 1e-5 over 50 iterations
+and stays on the same plateau.  The annealing macro
 legaliser measures overlap on integer-truncated rectangles
 (legalizer.cpp), so macros with fractional sizes may still overlap by less
 than one unit; the test asks for that and no more.
@@ -134,3 +139,36 @@ def test_2m_cells_learned_controller_against_plain():
         assert abs(gap) < 1e-2
     finally:
         gp.close()
+
+
+def test_2m_cells_200_macros_mixed_flow_target_007():
+    """Config 5 as specified: 2M std cells and 200 movable macros, 2048x2048
+    bins, target overflow 0.07, mGP -> mLG -> FILLERONLY -> cGP, plain
+    backtracking.  Each stage ends by the reference's own rule: mGP and cGP
+    when the overflow falls below 0.07 or after 1000 iterations
+    (nesterov.hpp:99-117), FILLERONLY after 20; the macro legaliser leaves no
+    overlap; cGP lowers the overflow mGP left."""
+    say("[config] 2M cells + 200 macros, 2048x2048 bins, mGP -> mLG -> FILLERONLY -> cGP to overflow 0.07")
+    t = time.perf_counter()
+    d = synthetic(2000000, 2100000, seed=5, num_io=1000, util=0.8, num_macros=200, name="cells2m_macros")
+    macros = d.node_is_macro()
+    die = (d.row_x1.max() - d.row_x0.min()) * (d.row_bottom[-1] + d.row_height[-1] - d.row_bottom[0])
+    macro_area = float((d.node_w[macros].astype(np.float64) * d.node_h[macros]).sum())
+    say(f"  design: {d.num_nodes} nodes, {d.num_pins} pins, {int(macros.sum())} macros covering "
+        f"{macro_area / die:.3f} of the die ({time.perf_counter() - t:.1f}s)")
+    assert int(macros.sum()) == 200 and macro_area <= 0.45 * die * (1 + 1e-6)
+    t = time.perf_counter()
+    out = place(d, 1.0, target_overflow=0.07, qp=False, legal=False, bins=2048, poisson_mode=2, method="bktrk",
+                verbose=True)
+    ext = macro_overlap_extent(d, out["positions"])
+    m, c = out["mgp"], out["cgp"]
+    say(f"  flow {time.perf_counter() - t:.1f}s: mGP {m['iterations']} it (overflow {m['final_overflow']:.4f}), "
+        f"mLG overlap {out['mlg']['final_overlap']} ({out['mlg']['seconds']:.1f}s), FILLERONLY "
+        f"{out['fillers']['iterations']} it, cGP {c['iterations']} it (overflow {c['final_overflow']:.4f}), "
+        f"HPWL {out['hpwl']:.6g}, macro overlap extent {ext:.3f}")
+    for r in (m, c):
+        assert r["final_overflow"] < 0.07 or r["iterations"] == 1001
+    assert out["mlg"]["macros"] == 200 and out["mlg"]["final_overlap"] == 0 and ext < 1.0
+    assert out["fillers"]["iterations"] == 20
+    assert c["final_overflow"] < m["final_overflow"]
+    assert np.isfinite(out["hpwl"]) and out["hpwl"] > 0
