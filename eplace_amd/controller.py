@@ -10,15 +10,20 @@ library's own forward / backward / Adam kernels (ep_ctrl_train).
 Training samples come from the optimiser log (EP_LOG_* columns): inputs are
 the logged features of iteration k, targets are what plain backtracking
 did from there:
-    t0 = min(0, log(1/L' at the accepted trial / plain proposal 1/L)), so the
-         controller only shrinks a step that plain backtracking would have
-         had to retry;
-    t1 = log(lambda after iteration k + c - 1 / lambda before iteration k)
-         - log(plain multiplier of iteration k): the extra penalty growth
-         that takes plain's growth over a lookahead of c iterations
-         (fractional c interpolates the last one).  c = 1 gives 0 (the
-         reference's updatePenaltyFactor); c > 1 trades HPWL for fewer
-         iterations.
+    t0 = log(1/L' at the accepted trial / plain proposal 1/L) where plain
+         backtracking retried the proposal (1/L' < 0.95 proposal), else 0:
+         the controller only shrinks a step that plain backtracking would
+         have had to retry;
+    t1 = log(f_k / 0.95), f_k the floor under the reference's multiplier:
+         with G_k plain's mean lambda growth per iteration over the next W
+         iterations, f_k = `floor` where G_k <= g_lo, no floor (0.95) where
+         G_k >= g_hi, linear in between, and likewise faded out unless HPWL
+         is between scale_lo and scale_hi times 350000 or more.  The reference's multiplier
+         1.05^(1 - dHPWL/350000) (eplace.cpp:846-883) uses an absolute HPWL
+         reference, so on large designs HPWL noise keeps it near 1 for
+         hundreds of iterations while lambda stalls; the floor lifts those
+         stalls and leaves iterations where plain's lambda already grows
+         alone.
 Iteration 0 is skipped (its plain proposal is the constant 1).
 """
 from __future__ import annotations
@@ -120,25 +125,29 @@ class Controller:
         return float(loss[0]), g
 
 
-def samples_from_log(log: np.ndarray, lookahead: float = 1.0):
+def samples_from_log(log: np.ndarray, floor: float = 1.01, window: int = 20, g_lo: float = 1.0,
+                     g_hi: float = 1.02, scale_lo: float = 300.0, scale_hi: float = 800.0):
     """(X [n, 12], Y [n, 2]) training pairs from a plain backtracking log."""
     log = np.asarray(log, np.float64)
-    whole = int(np.floor(lookahead))
-    frac = lookahead - whole
-    span = whole + (1 if frac > 0 else 0)
-    if lookahead < 1.0 or len(log) < 1 + span:
+    n = len(log)
+    if n < 2:
         return np.zeros((0, F), np.float32), np.zeros((0, NO), np.float32)
     with np.errstate(divide="ignore", invalid="ignore"):
-        lm = np.log(log[:, L.EP_LOG_LAMBDA] / log[:, L.EP_LOG_LAMBDA_BEFORE])  # per-iteration log multiplier
-        t0 = np.minimum(0.0, np.log(log[:, L.EP_LOG_STEP] / log[:, L.EP_LOG_STEP_PROP]))
-    ks = np.arange(1, len(log) - span + 1)
-    t1 = np.zeros(len(ks))
-    for j in range(whole):
-        t1 += lm[ks + j]
-    if frac > 0:
-        t1 += frac * lm[ks + whole]
+        r = log[:, L.EP_LOG_STEP] / log[:, L.EP_LOG_STEP_PROP]
+        # shorten only where plain backtracking retried (0.95 step > 1/L', nesterov.hpp:355)
+        t0 = np.where(r < 0.95, np.log(r), 0.0)
+        ks = np.arange(1, n)
+        last = np.minimum(ks + window - 1, n - 1)
+        growth = np.exp(np.log(log[last, L.EP_LOG_LAMBDA] / log[ks, L.EP_LOG_LAMBDA_BEFORE]) / (last - ks + 1))
+    w = np.clip((g_hi - growth) / (g_hi - g_lo), 0.0, 1.0)
+    # only where HPWL dwarfs the reference's absolute dHPWL constant
+    with np.errstate(divide="ignore", invalid="ignore"):
+        scale = np.log(log[ks, L.EP_LOG_HPWL] / 350000.0)
+    w = w * np.clip((scale - np.log(scale_lo)) / (np.log(scale_hi) - np.log(scale_lo)), 0.0, 1.0)
+    f = 0.95 + w * (floor - 0.95)
+    t1 = np.log(f / 0.95)
     X = log[ks, L.EP_LOG_FEATURES:L.EP_LOG_FEATURES + F]
-    Y = np.stack([t0[ks], t1 - lm[ks]], axis=1)
+    Y = np.stack([t0[ks], t1], axis=1)
     ok = np.all(np.isfinite(X), axis=1) & np.all(np.isfinite(Y), axis=1)
     return X[ok].astype(np.float32), Y[ok].astype(np.float32)
 

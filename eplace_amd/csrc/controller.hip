@@ -3,7 +3,8 @@
 // No counterpart in the reference; it drives method 2 of the optimiser (the
 // reference's backtracking Nesterov, nesterov.hpp:310-369, with the first
 // trial step proposed by the network and the penalty multiplier of
-// updatePenaltyFactor, eplace.cpp:846-883, scaled by the network's factor).
+// updatePenaltyFactor, eplace.cpp:846-883, held at or above the network's
+// floor).
 //
 // Features of iteration k, computed on the device at y_k (after the
 // combine at the current lambda and the plain step proposal 1/L):
@@ -12,17 +13,19 @@
 //   4 100 (HPWL_k - HPWL_{k-1}) / HPWL_k
 //   5 log(|grad WL| / (lambda_k |grad D|))
 //   6 log(s_{k-1} / s_0)         7 log(s_{k-1} / s_{k-2})   8 log(s_{k-2} / s_{k-3})
-//   9 k / 1000 (MAX_ITERATION)   10 log(proposal_k / s_{k-1})
-//   11 log(lambda_k / lambda_{k-1})
+//   9 k / 1000 (MAX_ITERATION)   10 log(HPWL_k / 350000)
+//   11 100 m_k, m the running mean (weight 1/16, from log 1.05) of the
+//      log of the reference's own multiplier 1.05^(1 - dHPWL/350000)
 // with s_j the accepted 1/L' of iteration j; terms that need missing history
-// are 0.  Every feature is free of the design's absolute scale (the
-// reference's penalty rule, which is not, is applied by the device itself).
+// are 0.  Features 10 and 11 carry the design's scale against the
+// reference's absolute dHPWL constant (DELTA_HPWL_REF, eplace.h:11), which
+// on large designs keeps the reference's multiplier near 1 for hundreds of
+// iterations.
 // Outputs: o0 = log(step / proposal), clamped to [log 1/4, 0] (the
-// controller only shortens the plain proposal), and
-// o1 = log of the factor applied on top of the reference's penalty
-// multiplier, clamped to [-0.05, 0.1]; the product stays inside the
-// reference's multiplier range [0.95, 1.05].  o1 = 0 reproduces the
-// reference.
+// controller only shortens the plain proposal), and o1, clamped to
+// [0, log(1.05/0.95)]: the iteration's multiplier is max(reference
+// multiplier, 0.95 e^o1), inside the reference's range [0.95, 1.05].
+// o0 = o1 = 0 reproduces the reference.
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -140,7 +143,7 @@ __global__ void __launch_bounds__(H) ctrl_step_kernel(DevScalars *sc, const floa
   __shared__ float x[F], h1[H], h2[H], out[NO];
   if (threadIdx.x < F) {
     const int i = threadIdx.x, k = sc->iter;
-    const double tau = sc->overflow, lam = sc->lambda, hp = sc->hpwl, prop = sc->step;
+    const double tau = sc->overflow, lam = sc->lambda, hp = sc->hpwl;
     double num = 1.0, den = 1.0;  // feature = log(num / den) when `ratio`
     bool ratio = true, have = true;
     float direct = 0.f;
@@ -155,8 +158,8 @@ __global__ void __launch_bounds__(H) ctrl_step_kernel(DevScalars *sc, const floa
       case 7: have = k >= 2; num = sc->step_hist[0]; den = sc->step_hist[1]; break;
       case 8: have = k >= 3; num = sc->step_hist[1]; den = sc->step_hist[2]; break;
       case 9: ratio = false; direct = (float)k / 1000.f; break;
-      case 10: have = k >= 1; num = prop; den = sc->step_hist[0]; break;
-      default: have = k > 0; num = lam; den = sc->lambda_before; break;
+      case 10: num = hp; den = 350000.0; break;  // HPWL in units of DELTA_HPWL_REF (eplace.h:11)
+      default: ratio = false; direct = (float)(100.0 * sc->ref_mult_ema); break;
     }
     const float f = ratio ? (have ? flog(log(num / den)) : 0.f) : direct;
     sc->feat[i] = f;
@@ -172,7 +175,7 @@ __global__ void __launch_bounds__(H) ctrl_step_kernel(DevScalars *sc, const floa
     sc->backtracks_mark = sc->backtracks;
     sc->trial = 0;
     sc->ctrl_active = active;
-    sc->lam_mult = 1.0;
+    sc->lam_floor = 0.95;
     sc->step_used = prop;
   }
   if (!active) return;
@@ -180,10 +183,11 @@ __global__ void __launch_bounds__(H) ctrl_step_kernel(DevScalars *sc, const floa
   if (threadIdx.x == 0) {
     const float o0 = out[0] * P[oYsd] + P[oYmu], o1 = out[1] * P[oYsd + 1] + P[oYmu + 1];
     const double s = fmin(fmax((double)o0, -1.3862943611198906), 0.0);
-    const double l = fmin(fmax((double)o1, -0.05), 0.1);
+    // floor under the reference's multiplier: 0.95 e^l, l in [0, log(1.05/0.95)]
+    const double l = fmin(fmax((double)o1, 0.0), 0.10008345855698253);
     if (isfinite(s) && isfinite(l)) {
       sc->step = sc->step_prop * exp(s);
-      sc->lam_mult = exp(l);
+      sc->lam_floor = 0.95 * exp(l);
     }
     sc->step_used = sc->step;
   }

@@ -756,7 +756,8 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
     h.hpwl_prev = h.hpwl;
     h.step_prop = h.step_used = 1;
     h.lambda_before = h.lambda;
-    h.lam_mult = 1;
+    h.lam_floor = 0.95;
+    h.ref_mult_ema = 0.04879016416943205;  // log 1.05: no stall seen yet
     h.backtracks_mark = h.trial = h.fallbacks = h.ctrl_active = 0;
     h.bb_short = 0;
   });

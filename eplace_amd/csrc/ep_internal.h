@@ -104,7 +104,10 @@ struct DevScalars {
   double lambda0, hpwl0, step0;       // at the start of the run / first accepted step
   double step_hist[3];                // accepted 1/L' of the last three iterations
   double tau_prev, hpwl_prev;         // at y_{k-1}
-  double step_prop, step_used, lambda_before, lam_mult;
+  double step_prop, step_used, lambda_before, lam_floor;
+  // running mean of log(reference multiplier), weight 1/16 per update: how
+  // fast the reference rule itself would grow lambda (a controller feature)
+  double ref_mult_ema;
   float feat[EP_CTRL_FEATURES];
   int backtracks_mark, trial, fallbacks, ctrl_active;
   int bb_short;                       // iterations that took the short Barzilai-Borwein step

@@ -326,7 +326,7 @@ __global__ void __launch_bounds__(kBlock) wl_group_kernel(WlClasses cls, LanePin
 
 // Sums the per-block HPWL/WL partials in a fixed order; optionally applies
 // updatePenaltyFactor (eplace.cpp:846-883) with FP64 HPWL and lambda
-// (lambda_update 1), times the controller's factor (lambda_update 2).  The
+// (lambda_update 1), or no lower than the controller's floor (lambda_update 2).  The
 // WA value is stored only when the wiring pass computed it (with_value).
 __global__ void __launch_bounds__(kFinish) reduce_wl_kernel(int nslots, double *partials, DevScalars *sc,
                                                             int lambda_update, int with_value) {
@@ -342,9 +342,10 @@ __global__ void __launch_bounds__(kFinish) reduce_wl_kernel(int nslots, double *
       double delta = hp - sc->last_hpwl;
       double mult = delta < 0.0 ? 1.05 : pow(1.05, -delta / 350000.0 + 1.0);
       mult = fmin(1.05, fmax(0.95, mult));
-      // the controller's factor of this iteration, kept inside the
-      // reference's own multiplier range
-      if (lambda_update == 2) mult = fmin(1.05, fmax(0.95, mult * sc->lam_mult));
+      sc->ref_mult_ema += (log(mult) - sc->ref_mult_ema) * (1.0 / 16.0);
+      // the controller's floor of this iteration (0.95: none), kept inside
+      // the reference's own multiplier range
+      if (lambda_update == 2) mult = fmin(1.05, fmax(mult, sc->lam_floor));
       sc->lambda *= mult;
       sc->last_hpwl = hp;
     }

@@ -3,10 +3,10 @@ evaluate it against plain backtracking on unseen designs.
 
     python -m eplace_amd.train_controller [--designs 64] [--out eplace_amd/controller_params.npy]
 
-Training designs: seeded synthetic designs of 50K-300K cells in the std-cell
-recipe (lognormal widths, 1K IO, 1.05 nets per cell, utilisation 0.7,
-target density 1.0), every fourth with 20 movable macros; the reference's
-bin rule.  Each is placed by plain backtracking Nesterov with the optimiser
+Training designs: seeded synthetic designs of 50K-300K cells and a set of
+0.5M-1.5M cells in the std-cell recipe (lognormal widths, 1K IO, 1.05 nets
+per cell, utilisation 0.7, target density 1.0), every fourth with 20
+movable macros; the reference's bin rule.  Each is placed by plain backtracking Nesterov with the optimiser
 log on; every logged iteration is a training pair (controller.samples_from_log).
 Evaluation: unseen seeds at 300K cells and 512x512 bins and at 1M cells
 (the reference's bin rule), plain vs learned, reporting iterations, full
@@ -31,11 +31,12 @@ def training_design(i: int, cells: int):
                      num_macros=macros, name=f"train{i}")
 
 
-def trajectories(n: int, lo: int, hi: int, poisson_mode: int = 2, verbose: bool = True):
-    """Plain backtracking logs of n training designs."""
-    rng = np.random.default_rng(7)
+def trajectories(n: int, lo: int, hi: int, poisson_mode: int = 2, verbose: bool = True, first: int = 0,
+                 rng_seed: int = 7):
+    """Plain backtracking logs of n training designs (seeds first .. first + n - 1)."""
+    rng = np.random.default_rng(rng_seed)
     logs, info = [], []
-    for i in range(n):
+    for i in range(first, first + n):
         cells = int(rng.integers(lo, hi + 1))
         d = training_design(i, cells)
         gp = EPlacer2D(d, 1.0, poisson_mode=poisson_mode)
@@ -50,8 +51,8 @@ def trajectories(n: int, lo: int, hi: int, poisson_mode: int = 2, verbose: bool 
     return logs, info
 
 
-def samples(logs, lookahead: float = 1.0):
-    pairs = [samples_from_log(g, lookahead) for g in logs]
+def samples(logs, **kw):
+    pairs = [samples_from_log(g, **kw) for g in logs]
     return np.concatenate([p[0] for p in pairs]), np.concatenate([p[1] for p in pairs])
 
 
@@ -82,8 +83,12 @@ def main(argv=None):
     ap.add_argument("--max-cells", type=int, default=300_000)
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--lookahead", type=float, default=1.5,
-                    help="penalty growth target: plain's over this many iterations per iteration")
+    ap.add_argument("--big-designs", type=int, default=12, help="training designs of 0.5M-1.5M cells")
+    ap.add_argument("--floor", type=float, default=1.01, help="penalty multiplier floor where plain's lambda stalls")
+    ap.add_argument("--window", type=int, default=20, help="iterations over which plain's lambda growth is measured")
+    ap.add_argument("--step-target", choices=["none", "retry"], default="none",
+                    help="step output target: none (keep the plain proposal) or retry (shorten where plain "
+                         "backtracking retried); at 1M cells 'retry' measured 1.43-1.46x against 1.77-1.85x")
     ap.add_argument("--eval-seeds", type=int, default=8)
     ap.add_argument("--eval-cells", type=int, default=300_000)
     ap.add_argument("--metric-seeds", type=int, default=3, help="also evaluate 1M-cell designs of these seeds")
@@ -92,8 +97,14 @@ def main(argv=None):
     a = ap.parse_args(argv)
     t = time.perf_counter()
     logs, info = trajectories(a.designs, a.min_cells, a.max_cells)
-    X, Y = samples(logs, a.lookahead)
-    print(f"[train] {len(X)} samples from {a.designs} designs in {time.perf_counter() - t:.1f}s", flush=True)
+    if a.big_designs:
+        big_logs, big_info = trajectories(a.big_designs, 500_000, 1_500_000, first=a.designs, rng_seed=8)
+        logs += big_logs
+        info += big_info
+    X, Y = samples(logs, floor=a.floor, window=a.window)
+    if a.step_target == "none":
+        Y[:, 0] = 0.0
+    print(f"[train] {len(X)} samples from {len(logs)} designs in {time.perf_counter() - t:.1f}s", flush=True)
     t = time.perf_counter()
     ctrl, losses = fit(X, Y, steps=a.steps, lr=a.lr)
     print(f"[train] {a.steps} Adam steps in {time.perf_counter() - t:.1f}s: loss {losses[0]:.4f} -> {losses[-1]:.4f}",
@@ -126,7 +137,8 @@ def main(argv=None):
     big = [(s, synthetic(1_000_000, 1_050_000, seed=2024 + s, num_io=1000, util=0.7, name=f"m{s}"), 0)
            for s in range(a.metric_seeds)]
     mv = run_set("1M", big)
-    summary = {"samples": int(len(X)), "lookahead": a.lookahead, "designs": info, "loss_first": float(losses[0]),
+    summary = {"samples": int(len(X)), "floor": a.floor, "window": a.window, "step_target": a.step_target,
+               "designs": info, "loss_first": float(losses[0]),
                "loss_last": float(losses[-1]), "eval_300k_512bins": ev, "eval_1m": mv}
     if a.report:
         with open(a.report, "w") as f:

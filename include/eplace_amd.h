@@ -111,7 +111,7 @@ enum {
 typedef struct ep_opt_params {
   int32_t method;           /* 0 vanilla (default), 1 backtracking (-bktrk),
                                2 learned: backtracking whose first trial step and
-                               a factor on the penalty multiplier come from
+                               a floor under the penalty multiplier come from
                                `controller`; a first trial that fails the
                                backtracking acceptance test continues as plain
                                backtracking; 3 Barzilai-Borwein (-bb): one move per
@@ -374,10 +374,11 @@ int ep_check_legal(const ep_design *design, const float *node_xy, double site_st
  * EP_CTRL_FEATURES -> EP_CTRL_HIDDEN -> EP_CTRL_HIDDEN -> EP_CTRL_OUTPUTS
  * (tanh hidden layers, FP32) evaluated on the device once per iteration of
  * method 2.  Inputs are the EP_LOG_FEATURES columns; outputs are
- * log(step / plain proposal), clamped to [log 1/4, 0], and the log of a
- * factor on the reference's penalty multiplier (eplace.cpp:846-883),
- * clamped to [-0.05, 0.1], the product kept in the reference's range
- * [0.95, 1.05]; zero outputs reproduce plain backtracking.
+ * log(step / plain proposal), clamped to [log 1/4, 0], and l, clamped to
+ * [0, log(1.05/0.95)], which sets a floor 0.95 e^l under the reference's
+ * penalty multiplier (eplace.cpp:846-883): the iteration multiplies lambda
+ * by max(reference multiplier, floor), which stays in the reference's
+ * range [0.95, 1.05]; zero outputs reproduce plain backtracking.
  *
  * Parameter vector (EP_CTRL_PARAMS floats): W1[H][F], b1[H], W2[H][H], b2[H],
  * W3[O][H], b3[O] (the EP_CTRL_WEIGHTS trained values), then the input

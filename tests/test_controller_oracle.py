@@ -51,17 +51,33 @@ def test_samples_from_log():
     rows[:, L.EP_LOG_STEP_PROP] = [1.0, 1.0, 6.0, 2.0, 1.0]
     rows[:, L.EP_LOG_LAMBDA_BEFORE] = [1.0, 1.0, 2.0, 4.0, 3.8]
     rows[:, L.EP_LOG_LAMBDA] = [1.0, 2.0, 2.1, 3.8, 4.0]
+    rows[:, L.EP_LOG_HPWL] = 350000.0 * 1000  # a large design: the floor applies in full
     X, Y = samples_from_log(rows)
     assert X.shape == (4, 12) and Y.shape == (4, 2)
     np.testing.assert_allclose(X[0], np.arange(12, 24))
     # step target: only shrinking is learned
     np.testing.assert_allclose(Y[:, 0], [0.0, np.log(0.5), 0.0, 0.0], atol=1e-7)
-    np.testing.assert_allclose(Y[:, 1], 0.0, atol=1e-7)  # lookahead 1: the reference rule itself
-    # lookahead 2: products of two consecutive multipliers; 1.5 halves the second
-    X2, Y2 = samples_from_log(rows, lookahead=2)
-    assert len(X2) == 3
-    np.testing.assert_allclose(Y2[:, 1], np.log([1.05, 0.95, 4.0 / 3.8]), atol=1e-6)
-    X3, Y3 = samples_from_log(rows, lookahead=1.5)
-    np.testing.assert_allclose(Y3[0, 1], 0.5 * np.log(1.05), rtol=1e-6)
+    # penalty floor over a window of one iteration: plain's own multiplier
+    # 2.0, 1.05, 0.95, 1.0526 -> no floor where it grows by 2% or more, the
+    # full floor 1.01 where it does not grow
+    full = np.log(1.01 / 0.95)
+    _, Y1 = samples_from_log(rows, window=1)
+    np.testing.assert_allclose(Y1[:, 1], [0.0, 0.0, full, 0.0], atol=1e-7)
+    # a window of two: mean growth sqrt(2.1), sqrt(1.9), 1.0 (4.0 / 4.0), and
+    # the last row's own multiplier
+    _, Y2 = samples_from_log(rows, window=2)
+    np.testing.assert_allclose(Y2[:, 1], [0.0, 0.0, full, 0.0], atol=1e-7)
+    # linear between no growth (full floor) and 2% growth (no floor)
+    rows[3, L.EP_LOG_LAMBDA] = 4.0 * 1.01
+    _, Y3 = samples_from_log(rows, window=1)
+    np.testing.assert_allclose(Y3[2, 1], np.log((0.95 + 0.5 * 0.06) / 0.95), rtol=1e-6)
+    # faded out by design scale: none at HPWL = 300 x 350000, half at the
+    # geometric middle of 300 and 800 (row 3: half the growth ramp as well)
+    rows[:, L.EP_LOG_HPWL] = 350000.0 * 300
+    _, Y4 = samples_from_log(rows, window=1)
+    np.testing.assert_allclose(Y4[:, 1], 0.0, atol=1e-7)
+    rows[:, L.EP_LOG_HPWL] = 350000.0 * np.sqrt(300 * 800)
+    _, Y5 = samples_from_log(rows, window=1)
+    np.testing.assert_allclose(Y5[2, 1], np.log((0.95 + 0.25 * 0.06) / 0.95), rtol=1e-6)
     xmu, xsd, ymu, ysd = normalization(X, Y)
     assert np.all(xsd > 0) and np.all(ysd > 0)
