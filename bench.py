@@ -86,7 +86,7 @@ PHASE_BYTES = {
 }
 
 PHASE_KERNELS = {
-    "scatter": "tile_hist_kernel + tile_keyscan_kernel + tile_place_kernel + tile_scatter_kernel + scatter_big_kernel",
+    "scatter": "tile_hist_kernel + tile_place_kernel + tile_scatter_kernel + scatter_big_kernel",
     "finalize": "finalize_density_kernel + finish_density_kernel",
     "wirelength": "wl_group_kernel<0,0,1> + wl_large_kernel<0,0,1>",
     "node_sums": "reduce_wl_kernel + node_wl_kernel",
@@ -161,21 +161,26 @@ def algorithmic_poisson_flops(info):
     return 6.0 * 2.0 * nx * ny * (nx + ny) / 2.0
 
 
-def roofline(name, seconds, info, poisson_mode=2):
+def roofline(name, seconds, info, poisson_mode=2, gemm_seconds=None):
     """Achieved rate of one evaluation phase against its roofline: HBM bytes
-    for the streaming phases, MFMA flops for the Poisson transforms."""
+    for the streaming phases, MFMA flops for the Poisson transforms.  For the
+    transforms the rate is that of the four GEMM launches alone (timed with
+    events around each launch, gemm_seconds per evaluation); the phase also
+    holds the kernels that split the data operands into planes."""
     O, N, P = info["num_objects"], info["num_nodes"], info["num_pins"]
     nb = info["bins_x"] * info["bins_y"]
     if name == "poisson":
         flops = algorithmic_poisson_flops(info)
         peak = FP32_MFMA_PEAK_TFLOPS if poisson_mode == 0 else BF16_MFMA_PEAK_TFLOPS
-        ach = flops / seconds / 1e12
+        gs = gemm_seconds if gemm_seconds else seconds
+        ach = flops / gs / 1e12
         issued = poisson_flops(info) * POISSON_PRODUCTS[poisson_mode]
         return {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
-                "traffic": per_launch(phase_traffic(name, info), 4), "kernel": POISSON_KERNEL[poisson_mode],
-                "launches": 4, "work_per_launch": flops / 4, "seconds_per_launch": seconds / 4,
+                "traffic": per_launch(phase_traffic("poisson_gemms", info), 4), "kernel": POISSON_KERNEL[poisson_mode],
+                "launches": 4, "work_per_launch": flops / 4, "seconds_per_launch": gs / 4,
+                "phase_seconds": seconds,
                 "work": "6 transform products of 2 N^3 flop per evaluation",
-                "issued_matrix_tflops": issued / seconds / 1e12,
+                "issued_matrix_tflops": issued / gs / 1e12,
                 "issued_note": "parity-folded GEMMs (half the algorithmic multiply-adds) times the MFMA products "
                                "issued per FP32 product"}
     byts = PHASE_BYTES[name](O, N, P, nb)
@@ -395,13 +400,14 @@ def main():
     r = learned(max_iterations=args.steps - 1).opt(profile=True)
     device_sync()
     dt_events = time.perf_counter() - t0
-    ph = r["phase_seconds"]
+    ph = dict(r["phase_seconds"])
+    gemm_s = ph.pop("poisson_gemms")
     # dominant phase: the largest measured time per evaluation among the
     # kernel phases (the combine also holds the wait for the second stream)
     kernel_phases = [k for k in ph if k != "combine"]
     dominant = max(kernel_phases, key=ph.get)
-    roof = roofline(dominant, ph[dominant], info, args.poisson_mode)
-    roof_all = {k: roofline(k, ph[k], info, args.poisson_mode) for k in ph}
+    roof = roofline(dominant, ph[dominant], info, args.poisson_mode, gemm_s)
+    roof_all = {k: roofline(k, ph[k], info, args.poisson_mode, gemm_s) for k in ph}
 
     conv = None
     headline = {}
@@ -530,7 +536,7 @@ def main():
         "roofline_phases": {k: {"bound": v["bound"], "achieved": v["achieved"], "unit": v["unit"],
                                 "frac": v["frac"], "frac_of_copy": v.get("frac_of_copy"), "traffic": v["traffic"]}
                             for k, v in roof_all.items()},
-        "phase_ms_per_evaluation": {k: v * 1e3 for k, v in ph.items()},
+        "phase_ms_per_evaluation": dict({k: v * 1e3 for k, v in ph.items()}, poisson_gemms=gemm_s * 1e3),
         "converge": conv,
         "qp": qp,
         "cpu_baseline": cpu,

@@ -532,7 +532,7 @@ static void enqueue_evaluation(Ctx &c, bool lambda_update, bool lipschitz, const
   // The transform GEMMs go first: their blocks (8 waves, 64 KB LDS) take one
   // CU each before the wiring kernel's blocks fill the rest; launched after
   // it, they waited for wiring blocks to retire.
-  launch_poisson(c, false);
+  launch_poisson(c, false, ev ? ev + kEvGemm : nullptr);
   if (ev) (void)hipEventRecord(ev[3], c.stream);
   (void)hipStreamWaitEvent(c.stream2, c.ev_fork, 0);
   if (ev) (void)hipEventRecord(ev[6], c.stream2);
@@ -606,6 +606,12 @@ void phase_times(const hipEvent_t *ev, double *out) {
     float ms = 0;
     (void)hipEventElapsedTime(&ms, ev[pairs[k][0]], ev[pairs[k][1]]);
     out[k] = ms * 1e-3;
+  }
+  out[kPhases] = 0;
+  for (int g = 0; g < 4; ++g) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ev[kEvGemm + 2 * g], ev[kEvGemm + 2 * g + 1]);
+    out[kPhases] += ms * 1e-3;
   }
 }
 
@@ -858,15 +864,15 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
   auto t1 = std::chrono::steady_clock::now();
   if (c.profiling) {
     c.profiling = false;
-    double acc[kPhases] = {};
+    double acc[kPhasesTimed] = {};
     size_t n = c.prof_used / kEvPerEval;
     for (size_t e = 0; e < n; ++e) {
-      double t[kPhases];
+      double t[kPhasesTimed];
       ep::phase_times(&c.prof_events[kEvPerEval * e], t);
-      for (int k = 0; k < kPhases; ++k) acc[k] += t[k];
+      for (int k = 0; k < kPhasesTimed; ++k) acc[k] += t[k];
     }
-    for (int k = 0; k < kPhases; ++k) o.phase_seconds[k] = acc[k];
-    o.phase_seconds[kPhases] = (double)n;
+    for (int k = 0; k < kPhasesTimed; ++k) o.phase_seconds[k] = acc[k];
+    o.phase_seconds[kPhasesTimed] = (double)n;
   }
   if (cap > 0) EP_HIP(hipMemcpy(o.log, dlog, (size_t)cap * EP_LOG_COLS * sizeof(double), hipMemcpyDeviceToHost));
   if (out) {

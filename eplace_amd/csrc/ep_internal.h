@@ -53,10 +53,15 @@ constexpr int kNetLaneMax = 32;  // nets above this get a wave each; the rest a 
 // in the reference's order).
 constexpr int kBigObjectBins = 64;
 constexpr int kXcds = 8;  // MI355X: workgroups go to the 8 XCDs round-robin
-constexpr int kEvPerEval = 9;  // phase events recorded per evaluation
+// phase events recorded per evaluation: 9 between the phases, then a start
+// and an end event around each of the 4 transform GEMM launches
+constexpr int kEvPerEval = 17;
+constexpr int kEvGemm = 9;
 // timed phases of an evaluation: scatter, density finalise, wiring kernels,
 // node sums of the pin gradients, Poisson solve, field gather, combine
 constexpr int kPhases = 7;
+// ... and the transform GEMM launches alone (inside the Poisson phase)
+constexpr int kPhasesTimed = kPhases + 1;
 
 // Renumbers blocks so that the blocks of each XCD cover one contiguous
 // 1/8 of the grid (block b runs on XCD b % 8).  Work ranges that are
@@ -166,7 +171,7 @@ struct Ctx {
   // 3 Poisson end, 4 field gather end, 5 combine end (stream), 6 wiring
   // start, 8 wiring kernels end, 7 node sums end (stream2)
   hipEvent_t ev[kEvPerEval] = {};
-  double last_timings[kPhases] = {};
+  double last_timings[kPhasesTimed] = {};
   // event pool for per-phase timing inside the optimiser loop (kEvPerEval per evaluation)
   std::vector<hipEvent_t> prof_events;
   // Captured evaluations (HIP graphs), one per distinct set of launch
@@ -286,7 +291,8 @@ void launch_reduce_wirelength(Ctx &c, int lambda_update, hipStream_t st = nullpt
 // ---- poisson.hip ----
 bool poisson_create(Ctx &c, std::string &err);
 void poisson_destroy(Ctx &c);
-void launch_poisson(Ctx &c, bool with_phi);
+// gemm_ev (optional): 8 events, recorded before and after each GEMM launch
+void launch_poisson(Ctx &c, bool with_phi, hipEvent_t *gemm_ev = nullptr);
 // ---- nesterov.hip ----
 void launch_positions_to_ref(Ctx &c, float2 *dst);
 // bb: the Barzilai-Borwein step (nesterov.hpp:231-302) instead of 1/L

@@ -783,8 +783,15 @@ static void set_a(PGemm &p, int z, const Mat &m, size_t off) {
   p.a_inv[z] = std::ldexp(1.f, -m.f16_exp);
 }
 
-void launch_poisson(Ctx &c, bool with_phi) {
+void launch_poisson(Ctx &c, bool with_phi, hipEvent_t *gev) {
   Poisson *q = c.poisson;
+  int g_launch = 0;
+  auto timed_gemm = [&](const PGemm &p, int batch) {
+    if (gev) (void)hipEventRecord(gev[2 * g_launch], c.stream);
+    gemm(c, p, batch);
+    if (gev) (void)hipEventRecord(gev[2 * g_launch + 1], c.stream);
+    ++g_launch;
+  };
   const int X = q->nxp, Y = q->nyp, Hxp = q->hxp, Hyp = q->hyp;
   const bool split_mode = q->mode != 0;
   const int np = q->mode == 3 ? 3 : 2;
@@ -825,7 +832,7 @@ void launch_poisson(Ctx &c, bool with_phi) {
     p.bscale = q->bscale;
   }
   int n_prev = wave_max_count(p, 2);
-  gemm(c, p, 2);
+  timed_gemm(p, 2);
   // 2. spec[u][v] = sum_x F_X[u][x] T1t[v][x] / (wx^2 + wy^2)
   p = PGemm{};
   set_a(p, 0, q->FXA, 0);
@@ -855,7 +862,7 @@ void launch_poisson(Ctx &c, bool with_phi) {
     p.bscale = q->bscale + 1;
   }
   n_prev = wave_max_count(p, 2);
-  gemm(c, p, 2);
+  timed_gemm(p, 2);
   // 3. Pt[y][u] = sum_v C_Y[v][y] spec[u][v] (mirror: e - o),
   //    Qt[y][u] = sum_v wy S_Y[v][y] spec[u][v] (mirror: o - e)
   p = PGemm{};
@@ -880,7 +887,7 @@ void launch_poisson(Ctx &c, bool with_phi) {
     p.bscale = q->bscale + 2;
   }
   n_prev = wave_max_count(p, 2);
-  gemm(c, p, 2);
+  timed_gemm(p, 2);
   // 4. Ex[x][y] = sum_u wx S_X[u][x] Pt[y][u] (mirror: o - e),
   //    Ey = sum_u C_X[u][x] Qt[y][u], phi = sum_u C_X[u][x] Pt[y][u] (e - o)
   p = PGemm{};
@@ -909,7 +916,7 @@ void launch_poisson(Ctx &c, bool with_phi) {
     set_b(p, 2, 0, 2 * Y, 2 * Hxp, 0);
     p.bscale = q->bscale + 3;
   }
-  gemm(c, p, with_phi ? 3 : 2);
+  timed_gemm(p, with_phi ? 3 : 2);
 }
 
 }  // namespace ep

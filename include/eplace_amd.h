@@ -123,10 +123,11 @@ typedef struct ep_opt_params {
   int32_t sync_every;       /* host polls the stop flag every this many iterations */
   int32_t log_capacity;     /* rows available in log (may be 0) */
   double *log;              /* optional [log_capacity][EP_LOG_COLS], columns EP_LOG_* */
-  double *phase_seconds;    /* optional [8]: summed time of the full evaluations per phase
+  double *phase_seconds;    /* optional [9]: summed time of the full evaluations per phase
                                (scatter, density finalise, wiring kernels, node sums of the pin
-                               gradients, poisson, field gather, combine) and the number of
-                               evaluations timed; HIP events on the streams the phases run on */
+                               gradients, poisson, field gather, combine), the transform GEMM
+                               launches alone (inside poisson), and the number of evaluations
+                               timed; HIP events on the streams the phases run on */
   const ep_ctrl *controller; /* method 2: the step / lambda controller (same device) */
 } ep_opt_params;
 
