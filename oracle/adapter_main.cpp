@@ -1,0 +1,170 @@
+// The reference's main flow (main/ePlace_main.cpp:35-219) with QP, mGP, mLG,
+// FILLERONLY and cGP taken over by eplace_amd/host/eplace_amd_adapter.h
+// (test infrastructure).
+//
+// This file is ours.  oracle/Makefile.ref links it with the reference's own,
+// unmodified Parser / PlaceCommon / PlaceDB sources (and the rest of the
+// objects the harness uses) and with eplace_amd/libeplace_amd.so, into
+// oracle/_ref/adapter_main.  It proves that the adapter compiles against the
+// reference's headers and that a PlaceDB filled by the reference's own parser
+// places through the library; the GPU tests compare what it prints with the
+// reference's own runs of the same designs.
+//
+//   adapter_main -aux F.aux -targetDensity TD [-noQP] [-loadpl F.pl]
+//                [-IPiteCount K] [-bktrk | -bb | -learned PARAMS.f32] [-LSE]
+//                [-mixed] -out DIR
+//
+// DIR/scalars.txt gets "key value" lines; DIR/node_centers.f32 the node
+// centres (dbNodes order, x/y float pairs) as the PlaceDB holds them at the end.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <exception>
+#include <fstream>
+#include <string>
+#include <sys/stat.h>
+#include <vector>
+
+#include "arghandler.h"
+#include "parser.h"
+#include "placedb.h"
+#include "eplace_amd_adapter.h"
+
+namespace {
+
+std::string arg_str(int argc, char **argv, const char *key, const char *def) {
+  for (int i = 1; i + 1 < argc; ++i)
+    if (argv[i][0] == '-' && strcmp(argv[i] + 1, key) == 0) return argv[i + 1];
+  return def;
+}
+bool arg_flag(int argc, char **argv, const char *key) {
+  for (int i = 1; i < argc; ++i)
+    if (argv[i][0] == '-' && strcmp(argv[i] + 1, key) == 0) return true;
+  return false;
+}
+
+std::ofstream g_scalars;
+void scalar(const char *k, double v) {
+  g_scalars.precision(17);
+  g_scalars << k << " " << v << "\n";
+  g_scalars.flush();
+}
+
+void run_stats(const char *stage, const ep_run_stats &s) {
+  std::string p(stage);
+  scalar((p + "_iterations").c_str(), s.iterations);
+  scalar((p + "_evaluations").c_str(), s.evaluations);
+  scalar((p + "_final_overflow").c_str(), s.final_overflow);
+  scalar((p + "_final_hpwl").c_str(), s.final_hpwl);
+  scalar((p + "_seconds").c_str(), s.seconds);
+}
+
+int run(int argc, char **argv) {
+  using namespace eplace_amd;
+  // ePlace_main.cpp:60-119: parse the Bookshelf design into a PlaceDB
+  const std::string aux = arg_str(argc, argv, "aux", "");
+  if (aux.empty()) {
+    fprintf(stderr, "adapter_main: -aux is required\n");
+    return 2;
+  }
+  const std::string out = arg_str(argc, argv, "out", ".");
+  mkdir(out.c_str(), 0755);
+  g_scalars.open(out + "/scalars.txt");
+  const size_t slash = aux.rfind('/');
+  gArg.Override("path", slash == std::string::npos ? std::string("./") : aux.substr(0, slash + 1));
+  BookshelfParser parser;
+  PlaceDB *db = new PlaceDB();
+  parser.ReadFile(aux, *db);
+  const std::string pl = arg_str(argc, argv, "loadpl", "");
+  if (!pl.empty()) parser.ReadPLFile(pl, *db, false);
+  scalar("hpwl_loaded", db->calcHPWL());
+
+  // ePlace_main.cpp:121-126
+  if (!arg_flag(argc, argv, "noQP")) {
+    const int k = atoi(arg_str(argc, argv, "IPiteCount", "20").c_str());
+    ep_qp_stats q = quadraticPlacementAMD(db, k);
+    scalar("qp_iterations", q.iterations);
+    scalar("qp_hpwl", db->calcHPWL());
+  }
+
+  // ePlace_main.cpp:141-168
+  const float td = (float)atof(arg_str(argc, argv, "targetDensity", "1.0").c_str());
+  scalar("hpwl_before_gp", db->calcHPWL());
+  EPlacerAMD gp(db, td, arg_flag(argc, argv, "LSE"));
+  // the library's HPWL of the same positions (penaltyFactorInitilization's
+  // lastHPWL): equal to the PlaceDB's only if the design arrays number the
+  // pins' modules as the library expects
+  scalar("ep_hpwl_initial", gp.stats().last_hpwl);
+  Step step = Step::Vanilla;
+  std::vector<float> ctrl;
+  if (arg_flag(argc, argv, "bktrk")) step = Step::Backtracking;
+  if (arg_flag(argc, argv, "bb")) step = Step::BarzilaiBorwein;
+  const std::string ctrlPath = arg_str(argc, argv, "learned", "");
+  if (!ctrlPath.empty()) {
+    std::ifstream f(ctrlPath, std::ios::binary);
+    ctrl.resize(EP_CTRL_PARAMS);
+    if (!f.read(reinterpret_cast<char *>(ctrl.data()), ctrl.size() * sizeof(float))) {
+      fprintf(stderr, "adapter_main: %s does not hold %d floats\n", ctrlPath.c_str(), (int)EP_CTRL_PARAMS);
+      return 2;
+    }
+    gp.setController(ctrl.data());
+    step = Step::Learned;
+  }
+  ep_run_stats m = gp.opt(step);
+  printf("mGP finished! Final HPWL: %d (%d iterations)\n", int(db->calcHPWL()), m.iterations);
+  run_stats("mgp", m);
+  scalar("mgp_hpwl", db->calcHPWL());
+
+  // ePlace_main.cpp:174-219 (macro designs only)
+  if (arg_flag(argc, argv, "mixed") && db->dbMacroCount > 0) {
+    ep_mlg_stats l = gp.macroLegalization(td);
+    scalar("mlg_final_overlap", l.final_overlap);
+    scalar("mlg_hpwl", db->calcHPWL());
+    std::vector<POS_3D> macroAt;
+    for (Module *n : db->dbNodes)
+      if (n->isMacro) macroAt.push_back(n->getCenter());
+    gp.switch2FillerOnly();
+    run_stats("fo", gp.opt(step));
+    gp.switch2cGP();
+    ep_run_stats c = gp.opt(step);
+    run_stats("cgp", c);
+    scalar("cgp_hpwl", db->calcHPWL());
+    // FILLERONLY moves fillers only and cGP keeps the macros (eplace.cpp:887-904)
+    int moved = 0;
+    size_t k = 0;
+    for (Module *n : db->dbNodes)
+      if (n->isMacro) {
+        moved += n->getCenter().x != macroAt[k].x || n->getCenter().y != macroAt[k].y;
+        ++k;
+      }
+    scalar("macros", (double)k);
+    scalar("macros_moved_after_mlg", moved);
+    printf("cGP finished! Final HPWL: %d (%d iterations)\n", int(db->calcHPWL()), c.iterations);
+  }
+
+  std::vector<float> xy;
+  for (Module *n : db->dbNodes) {
+    xy.push_back(n->getCenter().x);
+    xy.push_back(n->getCenter().y);
+  }
+  FILE *f = fopen((out + "/node_centers.f32").c_str(), "wb");
+  if (!f || fwrite(xy.data(), sizeof(float), xy.size(), f) != xy.size()) {
+    fprintf(stderr, "adapter_main: cannot write %s/node_centers.f32\n", out.c_str());
+    return 2;
+  }
+  fclose(f);
+  scalar("done", 1);
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  gArg.Init(argc, argv);
+  try {
+    return run(argc, argv);
+  } catch (const std::exception &e) {
+    fprintf(stderr, "adapter_main: %s\n", e.what());
+    return 1;
+  }
+}
