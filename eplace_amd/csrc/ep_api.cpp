@@ -158,7 +158,9 @@ int create(Ctx &c, const ep_design *d, const ep_params *p) {
   EP_HIP(hipStreamCreateWithFlags(&c.stream2, hipStreamNonBlocking));
   EP_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
   EP_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
-  for (auto &e : c.ev) EP_HIP(hipEventCreate(&e));
+  // timing events release at device scope only: a system-scope release writes
+  // back the L2 at every event and stretches the kernels between them
+  for (auto &e : c.ev) EP_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
   EP_HIP(hipHostMalloc((void **)&c.h_sc, sizeof(Ctx::HostScalars), hipHostMallocCoherent));
   c.h_sc->seq = 0;
   c.td = p->target_density;
@@ -474,6 +476,8 @@ int create(Ctx &c, const ep_design *d, const ep_params *p) {
             dev_zero(c, c.cur_grad, c.O) && dev_zero(c, c.sc, 1) && dev_zero(c, c.partials, kPartialTotal) &&
             dev_zero(c, c.tickets, 8);
   if (!ok) return EP_ERR_NOMEM;
+  if (hipDeviceGetAttribute(&c.wall_khz, hipDeviceAttributeWallClockRate, c.device) != hipSuccess || c.wall_khz <= 0)
+    c.wall_khz = 100000;
   std::string err;
   if (!poisson_create(c, err)) return c.err = err, EP_ERR_NOMEM;
   if (!tile_sort_create(c, err)) return c.err = err, EP_ERR_NOMEM;
@@ -498,7 +502,7 @@ void destroy(Ctx &c) {
                   c.net_start, c.pin_obj, c.pin_dx, c.pin_dy, c.pin_net, c.lane_obj, c.lane_off, c.large_nets, c.node_pin_start,
                   c.node_pin_lane, c.pin_grad, c.node_q, c.filler_q, c.term_map,
                   c.base_map, c.rho, c.phi, c.ex, c.ey, c.gd, c.gwl, c.gtot, c.cur_main, c.cur_ref, c.last_ref,
-                  c.last_grad, c.new_main, c.new_ref, c.cur_grad, c.sc, c.partials, c.tickets};
+                  c.last_grad, c.new_main, c.new_ref, c.cur_grad, c.sc, c.partials, c.tickets, c.gemm_stamps.t};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (auto &e : c.ev)
@@ -533,6 +537,7 @@ static void enqueue_evaluation(Ctx &c, bool lambda_update, bool lipschitz, const
   // CU each before the wiring kernel's blocks fill the rest; launched after
   // it, they waited for wiring blocks to retire.
   launch_poisson(c, false, ev ? ev + kEvGemm : nullptr);
+  if (ev) (void)hipEventRecord(ev[kEvMarker], c.stream);
   if (ev) (void)hipEventRecord(ev[3], c.stream);
   (void)hipStreamWaitEvent(c.stream2, c.ev_fork, 0);
   if (ev) (void)hipEventRecord(ev[6], c.stream2);
@@ -594,6 +599,10 @@ void evaluate(Ctx &c, bool lambda_update, bool lipschitz, const float2 *ref_now,
     hipGraphExec_t g = eval_graph(c, lambda_update, lipschitz, ref_now, ref_prev, grad_prev);
     if (g && hipGraphLaunch(g, c.stream) == hipSuccess) return;
   }
+  // profiled: the device waits until the whole evaluation is enqueued, so the
+  // phases run back to back as in a graph replay and the events time kernels,
+  // not the host's submissions
+  if (ev && !timed) launch_hold(c, c.prof_hold_ticks);
   enqueue_evaluation(c, lambda_update, lipschitz, ref_now, ref_prev, grad_prev, ev);
 }
 
@@ -607,11 +616,15 @@ void phase_times(const hipEvent_t *ev, double *out) {
     (void)hipEventElapsedTime(&ms, ev[pairs[k][0]], ev[pairs[k][1]]);
     out[k] = ms * 1e-3;
   }
+  // the GEMM launches: each start/end pair less the cost of one event, timed
+  // by the pair that follows the last GEMM's end event
+  float marker = 0;
+  (void)hipEventElapsedTime(&marker, ev[kEvGemm + 7], ev[kEvMarker]);
   out[kPhases] = 0;
   for (int g = 0; g < 4; ++g) {
     float ms = 0;
     (void)hipEventElapsedTime(&ms, ev[kEvGemm + 2 * g], ev[kEvGemm + 2 * g + 1]);
-    out[kPhases] += ms * 1e-3;
+    out[kPhases] += (ms - marker) * 1e-3;
   }
 }
 
@@ -735,11 +748,16 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
     need = std::min<size_t>(need, (size_t)kEvPerEval * 20000);
     while (c.prof_events.size() < need) {
       hipEvent_t e;
-      EP_HIP(hipEventCreate(&e));
+      EP_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
       c.prof_events.push_back(e);
     }
     c.prof_used = 0;
     c.profiling = true;
+    // 0.4 ms of the wall clock (its rate in kHz) ahead of each timed evaluation
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c.device) != hipSuccess || khz <= 0)
+      khz = 100000;
+    c.prof_hold_ticks = (unsigned long long)khz * 4 / 10;
   }
   auto t0 = std::chrono::steady_clock::now();
   // init (nesterov.hpp:207-214): a_0 = 1, main = current positions
@@ -766,8 +784,13 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
     h.ref_mult_ema = 0.04879016416943205;  // log 1.05: no stall seen yet
     h.backtracks_mark = h.trial = h.fallbacks = h.ctrl_active = 0;
     h.bb_short = 0;
+    h.gemm_ticks = 0;
+    h.gemm_launches = 0;
   });
   if (rc) return rc;
+  // GEMM launches before the run are not the run's: all stamps zero reads as folded
+  EP_HIP(hipMemsetAsync(c.gemm_stamps.t, 0, (size_t)4 * kGemmStamped * c.gemm_stamps.stride * sizeof(unsigned long long),
+                        c.stream));
   launch_positions_to_ref(c, c.cur_main);
   // stop check before the first step uses tau of the last evaluation
   launch_done_check_ex(c, 0, nullptr, 0, 0);
@@ -884,6 +907,8 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
     out->seconds = std::chrono::duration<double>(t1 - t0).count();
     out->controller_fallbacks = h.fallbacks;
     out->bb_short_steps = h.bb_short;
+    out->gemm_seconds = (double)h.gemm_ticks / (1e3 * c.wall_khz);
+    out->gemm_launches = h.gemm_launches;
   }
   return EP_OK;
 }
@@ -1087,6 +1112,9 @@ int ep_poisson(ep_ctx *x, const float *rho, float *phi, float *ex, float *ey) {
   EP_HIP(hipMemcpy(c.rho, buf.data(), buf.size() * 4, hipMemcpyHostToDevice));
   ep::launch_poisson(c, phi != nullptr);
   if ((rc = ep::sync(c))) return rc;
+  // this solve's launches belong to no run
+  EP_HIP(hipMemset(c.gemm_stamps.t, 0,
+                   (size_t)4 * ep::kGemmStamped * c.gemm_stamps.stride * sizeof(unsigned long long)));
   auto down = [&](float *dst, const float *src) -> int {
     EP_HIP(hipMemcpy(buf.data(), src, buf.size() * 4, hipMemcpyDeviceToHost));
     for (int i = 0; i < c.nx; ++i) std::memcpy(dst + (size_t)i * c.ny, &buf[(size_t)i * ld], (size_t)c.ny * 4);

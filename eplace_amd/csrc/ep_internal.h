@@ -53,10 +53,31 @@ constexpr int kNetLaneMax = 32;  // nets above this get a wave each; the rest a 
 // in the reference's order).
 constexpr int kBigObjectBins = 64;
 constexpr int kXcds = 8;  // MI355X: workgroups go to the 8 XCDs round-robin
-// phase events recorded per evaluation: 9 between the phases, then a start
-// and an end event around each of the 4 transform GEMM launches
-constexpr int kEvPerEval = 17;
+// phase events recorded per evaluation: 9 between the phases, a start and an
+// end event around each of the 4 transform GEMM launches, and one recorded
+// right after the last GEMM's end event: the two bracket nothing but the
+// processing of one event on the stream, which every event pair around a
+// kernel also holds and which is subtracted from the GEMM times
+constexpr int kEvPerEval = 18;
+// transform GEMM launches per Poisson solve
+constexpr int kGemmStamped = 4;
+
+// Wall-clock stamps of the transform GEMM launches.  Every block of launch L
+// of an evaluation stores its start and its end (plain stores, one pair per
+// block) in the stamp set of the evaluation count's parity,
+// t[2 ((parity kGemmStamped + L) stride + block)].  The WL reduction of the
+// next evaluation, on the second stream and off the critical path, reduces
+// the other set to (first start, last end) per launch while this
+// evaluation's GEMMs write theirs, and marks a launch folded by clearing its
+// block 0 end, which the next launch into that set rewrites.
+struct GemmStamps {
+  unsigned long long *t;     // [2][kGemmStamped][stride][2]
+  const int *evals;          // the evaluation count (DevScalars::evals)
+  int stride;                // blocks reserved per launch
+  int nblk[kGemmStamped];    // blocks of each launch of an evaluation's solve
+};
 constexpr int kEvGemm = 9;
+constexpr int kEvMarker = 17;
 // timed phases of an evaluation: scatter, density finalise, wiring kernels,
 // node sums of the pin gradients, Poisson solve, field gather, combine
 constexpr int kPhases = 7;
@@ -117,6 +138,10 @@ struct DevScalars {
   int backtracks_mark, trial, fallbacks, ctrl_active;
   int bb_short;                       // iterations that took the short Barzilai-Borwein step
   int pad1;
+  // transform GEMM launches folded by the density finisher: summed wall-clock
+  // ticks from each launch's first block start to its last block's end
+  unsigned long long gemm_ticks;
+  int gemm_launches, pad2;
 };
 
 // Slots of the partial-sum buffer (doubles).
@@ -185,6 +210,7 @@ struct Ctx {
   std::vector<EvalGraph> eval_graphs;
   bool graphs_ok = true;  // cleared if capture or instantiation fails
   size_t prof_used = 0;
+  unsigned long long prof_hold_ticks = 0;
   bool profiling = false;
 
   // sizes
@@ -251,6 +277,8 @@ struct Ctx {
   DevScalars *sc = nullptr;
   double *partials = nullptr;
   unsigned int *tickets = nullptr;
+  GemmStamps gemm_stamps = {};  // set up with the transforms (poisson.hip)
+  int wall_khz = 100000;        // wall clock rate
   Poisson *poisson = nullptr;
   TileSort tiles;
 
@@ -302,6 +330,7 @@ void launch_lambda_init(Ctx &c);
 void launch_set_positions(Ctx &c, const float2 *xy_stage, const int32_t *stage_idx, int64_t n);
 void launch_copy_float2(Ctx &c, float2 *dst, const float2 *src, int64_t n);
 void launch_publish_scalars(Ctx &c, unsigned int seq);
+void launch_hold(Ctx &c, unsigned long long ticks);
 void launch_done_check_ex(Ctx &c, int next_iter, double *log, int log_cap, int set_iter);
 void launch_nesterov_move_ex(Ctx &c, const float2 *ref, const float2 *grad, const float2 *main_in,
                              float2 *main_out, float2 *ref_unclamped_out, float2 *save_ref, float2 *save_grad);

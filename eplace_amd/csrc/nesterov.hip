@@ -231,6 +231,21 @@ void launch_publish_scalars(Ctx &c, unsigned int seq) {
   hipLaunchKernelGGL(publish_scalars_kernel, dim3(1), dim3(64), 0, c.stream, c.sc, c.h_sc, seq);
 }
 
+// Holds the stream for `ticks` periods of the constant-rate wall clock, so
+// that the host has enqueued a whole evaluation before the device starts it
+// when the evaluation's phases are timed with events (otherwise an event can
+// be stamped while the device waits for the host to submit the next kernel).
+// The loop is also bounded by a count of sleeps, so it ends even if the
+// clock did not advance.
+__global__ void __launch_bounds__(64) hold_kernel(unsigned long long ticks) {
+  const unsigned long long t0 = wall_clock64();
+  for (int i = 0; i < 100000 && wall_clock64() - t0 < ticks; ++i) __builtin_amdgcn_s_sleep(8);
+}
+
+void launch_hold(Ctx &c, unsigned long long ticks) {
+  hipLaunchKernelGGL(hold_kernel, dim3(1), dim3(64), 0, c.stream, ticks);
+}
+
 
 
 }  // namespace ep

@@ -129,6 +129,9 @@ struct PGemm {
   int ldbp;
   const float *bscale;  // mode 2: the power of two split_kernel scaled B by
   float *out_max;       // mode 2 (optional): one max |C| per wave of the launch
+  unsigned long long *stamp;  // optional: per-block (start, end) on the wall clock, this launch's
+  const int *evals;           // slots of parity 0; parity 1 at stamp_set further
+  int stamp_set;
 };
 
 constexpr int PBM = 64, PBN = 64, PK = 32;
@@ -429,6 +432,12 @@ __global__ void __launch_bounds__(kGroups * kGroupThreads) pgemm_kernel(PGemm p)
   typedef typename std::conditional<MODE != 0, SmemS<MODE == 0 ? 1 : MODE>, Smem32>::type SM;
   static_assert(sizeof(SM) >= 16 * kGroupThreads * sizeof(float), "reduction buffer");
   __shared__ __attribute__((aligned(16))) SM s[kGroups];
+  const int blk = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  unsigned long long *stamp = nullptr;
+  if (p.stamp && threadIdx.x == 0) {
+    stamp = p.stamp + (size_t)(*p.evals & 1) * p.stamp_set + 2 * blk;
+    stamp[0] = wall_clock64();
+  }
   const int z = blockIdx.z;
   const float *__restrict__ B = p.B[z];
   float *__restrict__ C = p.C[z];
@@ -497,11 +506,9 @@ __global__ void __launch_bounds__(kGroups * kGroupThreads) pgemm_kernel(PGemm p)
       }
   if (MODE == 2 && p.out_max) {
     for (int off = 32; off > 0; off >>= 1) wmax = fmaxf(wmax, __shfl_xor(wmax, off, 64));
-    if (lane == 0) {
-      const int blk = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-      p.out_max[blk * (kGroupThreads / 64) + wave] = wmax;
-    }
+    if (lane == 0) p.out_max[blk * (kGroupThreads / 64) + wave] = wmax;
   }
+  if (stamp) stamp[1] = wall_clock64();
 }
 
 // One stage's data operand split into planes for the split-mode GEMMs:
@@ -748,6 +755,18 @@ bool poisson_create(Ctx &c, std::string &err) {
             zeros(q->T1t, (size_t)2 * Hyp * X) && zeros(q->spec, (size_t)2 * Hxp * 2 * Hyp) &&
             zeros(q->PQt, (size_t)2 * Y * 2 * Hxp) && zeros(c.rho, (size_t)X * Y) && zeros(c.phi, (size_t)X * Y) &&
             zeros(c.ex, (size_t)X * Y) && zeros(c.ey, (size_t)X * Y);
+  // wall-clock stamps: the blocks of each stage's launch (batch 2; the
+  // standalone solve with phi adds a third batch to stage 4)
+  GemmStamps &st = c.gemm_stamps;
+  st.nblk[0] = (X / PBN) * (Hyp / PBM) * 2;
+  st.nblk[1] = (2 * Hyp / PBN) * (Hxp / PBM) * 2;
+  st.nblk[2] = (2 * Hxp / PBN) * (Hyp / PBM) * 2;
+  st.nblk[3] = (Y / PBN) * (Hxp / PBM) * 2;
+  st.stride = std::max(std::max(st.nblk[0], st.nblk[1]), std::max(st.nblk[2], st.nblk[3] / 2 * 3));
+  const size_t nstamp = (size_t)4 * kGemmStamped * st.stride;
+  st.evals = &c.sc->evals;
+  ok = ok && hipMalloc(&st.t, nstamp * sizeof(unsigned long long)) == hipSuccess &&
+       hipMemset(st.t, 0, nstamp * sizeof(unsigned long long)) == hipSuccess;
   if (!ok) err = "poisson_create: device allocation failed";
   // per-wave maxima of the largest launch (stages 1 and 3: (X/64)(Hyp/64) and
   // (2Hxp/64)(Hyp/64) blocks per batch, two batches, four waves each)
@@ -768,6 +787,8 @@ void poisson_destroy(Ctx &c) {
   if (q->bsplit) (void)hipFree(q->bsplit);
   for (float *p : ptrs)
     if (p) (void)hipFree(p);
+  if (c.gemm_stamps.t) (void)hipFree(c.gemm_stamps.t);
+  c.gemm_stamps = GemmStamps{};
   delete q;
   c.poisson = nullptr;
 }
@@ -786,7 +807,10 @@ static void set_a(PGemm &p, int z, const Mat &m, size_t off) {
 void launch_poisson(Ctx &c, bool with_phi, hipEvent_t *gev) {
   Poisson *q = c.poisson;
   int g_launch = 0;
-  auto timed_gemm = [&](const PGemm &p, int batch) {
+  auto timed_gemm = [&](PGemm &p, int batch) {
+    p.stamp = c.gemm_stamps.t + (size_t)2 * g_launch * c.gemm_stamps.stride;
+    p.evals = c.gemm_stamps.evals;
+    p.stamp_set = 2 * kGemmStamped * c.gemm_stamps.stride;
     if (gev) (void)hipEventRecord(gev[2 * g_launch], c.stream);
     gemm(c, p, batch);
     if (gev) (void)hipEventRecord(gev[2 * g_launch + 1], c.stream);

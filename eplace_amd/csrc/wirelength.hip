@@ -328,8 +328,58 @@ __global__ void __launch_bounds__(kBlock) wl_group_kernel(WlClasses cls, LanePin
 // updatePenaltyFactor (eplace.cpp:846-883) with FP64 HPWL and lambda
 // (lambda_update 1), or no lower than the controller's floor (lambda_update 2).  The
 // WA value is stored only when the wiring pass computed it (with_value).
+// Spans of the previous evaluation's transform GEMM launches, from their
+// per-block wall-clock stamps (the stamp set of the other parity): each
+// launch's first block start to its last block end, added to the run's sum.
+__device__ void fold_gemm_stamps(const GemmStamps &st, DevScalars *sc) {
+  __shared__ unsigned long long fmin_[kGemmStamped][kFinish / 64], fmax_[kGemmStamped][kFinish / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  unsigned long long *set = st.t + (size_t)((sc->evals - 1) & 1) * 2 * kGemmStamped * st.stride;
+  unsigned long long mark[kGemmStamped], mn[kGemmStamped], mx[kGemmStamped];
+#pragma unroll
+  for (int L = 0; L < kGemmStamped; ++L) mark[L] = set[(size_t)2 * L * st.stride + 1];
+#pragma unroll
+  for (int L = 0; L < kGemmStamped; ++L) {
+    const unsigned long long *t = set + (size_t)2 * L * st.stride;
+    mn[L] = ~0ull;
+    mx[L] = 0;
+    if (mark[L] != 0)
+      for (int b = threadIdx.x; b < st.nblk[L]; b += blockDim.x) {
+        mn[L] = min(mn[L], t[2 * b]);
+        mx[L] = max(mx[L], t[2 * b + 1]);
+      }
+  }
+#pragma unroll
+  for (int L = 0; L < kGemmStamped; ++L) {
+    for (int off = 32; off > 0; off >>= 1) {
+      mn[L] = min(mn[L], __shfl_xor(mn[L], off, 64));
+      mx[L] = max(mx[L], __shfl_xor(mx[L], off, 64));
+    }
+    if (lane == 0) {
+      fmin_[L][wave] = mn[L];
+      fmax_[L][wave] = mx[L];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int L = 0; L < kGemmStamped; ++L) {
+      unsigned long long a = ~0ull, z = 0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+        a = min(a, fmin_[L][w]);
+        z = max(z, fmax_[L][w]);
+      }
+      if (z > a) {
+        sc->gemm_ticks += z - a;
+        sc->gemm_launches += 1;
+        set[(size_t)2 * L * st.stride + 1] = 0;
+      }
+    }
+  }
+}
+
 __global__ void __launch_bounds__(kFinish) reduce_wl_kernel(int nslots, double *partials, DevScalars *sc,
-                                                            int lambda_update, int with_value) {
+                                                            int lambda_update, int with_value, GemmStamps st) {
+  if (st.t) fold_gemm_stamps(st, sc);
   if (sc->done) return;
   const double *const src[2] = {partials + kPartialWL, partials + kPartialWL + 1};
   double sum[2];
@@ -403,7 +453,7 @@ void launch_reduce_wirelength(Ctx &c, int lambda_update, hipStream_t st, bool wi
   if (!st) st = c.stream;
   int n = g_small_blocks(c) + (c.num_large ? g_large_blocks(c) : 0);
   hipLaunchKernelGGL(reduce_wl_kernel, dim3(1), dim3(kFinish), 0, st, n, c.partials, c.sc, lambda_update,
-                     with_value ? 1 : 0);
+                     with_value ? 1 : 0, c.gemm_stamps);
 }
 
 }  // namespace ep

@@ -127,7 +127,10 @@ typedef struct ep_opt_params {
                                (scatter, density finalise, wiring kernels, node sums of the pin
                                gradients, poisson, field gather, combine), the transform GEMM
                                launches alone (inside poisson), and the number of evaluations
-                               timed; HIP events on the streams the phases run on */
+                               timed; HIP events on the streams the phases run on.  Each timed
+                               evaluation is launched kernel by kernel behind a 0.4 ms hold
+                               kernel, so the host has enqueued all of it before the device
+                               starts it; iterates are unchanged, the run's wall time is not */
   const ep_ctrl *controller; /* method 2: the step / lambda controller (same device) */
 } ep_opt_params;
 
@@ -141,6 +144,12 @@ typedef struct ep_run_stats {
   double seconds;           /* wall time of the loop */
   int32_t controller_fallbacks; /* method 2: first trials that failed the acceptance test */
   int32_t bb_short_steps;   /* method 3: iterations that took the short BB step */
+  double gemm_seconds;      /* summed duration of the transform GEMM launches of the run's
+                               evaluations, each from its first block's start to its last
+                               wave's end on the device's wall clock (the last evaluation's
+                               launches are not included) */
+  int32_t gemm_launches;    /* launches summed in gemm_seconds */
+  int32_t pad;
 } ep_run_stats;
 
 /* Which device array ep_read/ep_write address (parity and debugging). */
