@@ -433,10 +433,10 @@ __global__ void __launch_bounds__(kBlock) gather_combine_kernel(
     float2 *gtot, int recompute, int lipschitz, const uint8_t *__restrict__ in_stage,
     const float2 *__restrict__ ref_now, const float2 *__restrict__ ref_prev,
     const float2 *__restrict__ grad_prev, double *partials, float2 *pos_out, float2 *gtot_out,
-    const DevScalars *sc) {
+    int at_next, const DevScalars *sc) {
   if (sc->done) return;
   int o = xcd_block() * blockDim.x + threadIdx.x;
-  double ln = 0, ld = 0, nw = 0, nd = 0, sy = 0;
+  double ln = 0, ld = 0, nw = 0, nd = 0, sy = 0, ln2 = 0, ld2 = 0;
   if (o < O) {
     uint8_t k = kind[o];
     const float2 d = gd[o];  // from the tile / big-object gather
@@ -465,9 +465,25 @@ __global__ void __launch_bounds__(kBlock) gather_combine_kernel(
       t.y = pre * (lam * d.y - wl.y);
     }
     gtot[o] = t;
+    // at_next: the same positions' gradient at the staged lambda
+    // (updatePenaltyFactor of this trial), which the combine-only pass after
+    // an accepted trial would compute: it becomes g_{k+1}, and its Lipschitz
+    // sums against y_k, g_k at the clamped positions give the next step
+    float2 t2 = t;
+    if (at_next) {
+      const float lam2 = (float)sc->lambda_next;
+      const float pre2 = 1.f / fmaxf(1.0f, (float)npins[o] + lam2 * area[o]);
+      if (k == KIND_FILLER) {
+        t2.x = pre2 * lam2 * d.x;
+        t2.y = pre2 * lam2 * d.y;
+      } else {
+        t2.x = pre2 * (lam2 * d.x - wl.x);
+        t2.y = pre2 * (lam2 * d.y - wl.y);
+      }
+    }
     // the optimiser's copies of y_k and g_k for the next trials
     if (pos_out) pos_out[o] = pos[o];
-    if (gtot_out) gtot_out[o] = t;
+    if (gtot_out) gtot_out[o] = t2;
     if (lipschitz && in_stage[o]) {
       float2 gp = grad_prev[o];
       float2 rn = ref_now ? ref_now[o] : pos[o];
@@ -479,9 +495,28 @@ __global__ void __launch_bounds__(kBlock) gather_combine_kernel(
       nw = (double)wl.x * wl.x + (double)wl.y * wl.y;
       nd = (double)d.x * d.x + (double)d.y * d.y;
       sy = dxx * dgx + dxy * dgy;
+      if (at_next) {
+        const float2 pc = pos[o];
+        double egx = (double)t2.x - gp.x, egy = (double)t2.y - gp.y;
+        double exx = (double)pc.x - rp.x, exy = (double)pc.y - rp.y;
+        ln2 = egx * egx + egy * egy;
+        ld2 = exx * exx + exy * exy;
+      }
     }
   }
-  if (lipschitz) {
+  if (lipschitz && at_next) {
+    double v[7] = {ln, ld, nw, nd, sy, ln2, ld2};
+    block_sum_n(v);
+    if (threadIdx.x == 0) {
+      partials[kPartialLip + 2 * blockIdx.x] = v[0];
+      partials[kPartialLip + 2 * blockIdx.x + 1] = v[1];
+      partials[kPartialNorm + 2 * blockIdx.x] = v[2];
+      partials[kPartialNorm + 2 * blockIdx.x + 1] = v[3];
+      partials[kPartialSY + 2 * blockIdx.x] = v[4];
+      partials[kPartialLipNext + 2 * blockIdx.x] = v[5];
+      partials[kPartialLipNext + 2 * blockIdx.x + 1] = v[6];
+    }
+  } else if (lipschitz) {
     double v[5] = {ln, ld, nw, nd, sy};
     block_sum_n(v);
     if (threadIdx.x == 0) {
@@ -646,11 +681,11 @@ void launch_field_gather(Ctx &c) {
 }
 
 void launch_combine(Ctx &c, bool recompute, bool lipschitz, const float2 *ref_now, const float2 *ref_prev,
-                    const float2 *grad_prev, float2 *pos_out, float2 *gtot_out) {
+                    const float2 *grad_prev, float2 *pos_out, float2 *gtot_out, bool at_next_lambda) {
   hipLaunchKernelGGL(gather_combine_kernel, dim3(object_blocks(c.O)), dim3(kBlock), 0, c.stream, c.O, c.N,
                      c.pos, c.kind, c.npins, c.area, c.node_pin_start, c.node_pin_lane, c.pin_grad, c.gd, c.gwl,
                      c.gtot, recompute ? 1 : 0, lipschitz ? 1 : 0, c.in_stage, ref_now, ref_prev, grad_prev,
-                     c.partials, pos_out, gtot_out, c.sc);
+                     c.partials, pos_out, gtot_out, at_next_lambda ? 1 : 0, c.sc);
 }
 
 void launch_gather_combine(Ctx &c, bool recompute, bool lipschitz, const float2 *ref_now,

@@ -525,8 +525,12 @@ void destroy(Ctx &c) {
 // and the node sums of their pin gradients need only gamma (set by the
 // density finalise), so they run on the second stream beside the Poisson
 // solve and the field gather; the combine waits for them.
+// A backtracking trial (next_lambda 1, or 2 with the controller's floor)
+// also stages its updatePenaltyFactor and writes the gradient at that lambda
+// to gtot_out with the positions to pos_out (see gather_combine_kernel).
 static void enqueue_evaluation(Ctx &c, bool lambda_update, bool lipschitz, const float2 *ref_now,
-                               const float2 *ref_prev, const float2 *grad_prev, hipEvent_t *ev) {
+                               const float2 *ref_prev, const float2 *grad_prev, int next_lambda, float2 *pos_out,
+                               float2 *gtot_out, hipEvent_t *ev) {
   if (ev) (void)hipEventRecord(ev[0], c.stream);
   launch_scatter(c);
   if (ev) (void)hipEventRecord(ev[1], c.stream);
@@ -543,14 +547,14 @@ static void enqueue_evaluation(Ctx &c, bool lambda_update, bool lipschitz, const
   if (ev) (void)hipEventRecord(ev[6], c.stream2);
   launch_wirelength(c, true, c.stream2);
   if (ev) (void)hipEventRecord(ev[8], c.stream2);
-  launch_reduce_wirelength(c, lambda_update ? 1 : 0, c.stream2);
+  launch_reduce_wirelength(c, next_lambda ? 2 + next_lambda : lambda_update ? 1 : 0, c.stream2);
   launch_node_wl(c, c.stream2);
   if (ev) (void)hipEventRecord(ev[7], c.stream2);
   (void)hipEventRecord(c.ev_join, c.stream2);
   launch_field_gather(c);
   if (ev) (void)hipEventRecord(ev[4], c.stream);
   (void)hipStreamWaitEvent(c.stream, c.ev_join, 0);
-  launch_combine(c, false, lipschitz, ref_now, ref_prev, grad_prev);
+  launch_combine(c, false, lipschitz, ref_now, ref_prev, grad_prev, pos_out, gtot_out, next_lambda != 0);
   if (ev) (void)hipEventRecord(ev[5], c.stream);
 }
 
@@ -558,10 +562,12 @@ static void enqueue_evaluation(Ctx &c, bool lambda_update, bool lipschitz, const
 // second stream joins the capture through the fork event and leaves it at
 // the join).  nullptr when capture is not available.
 static hipGraphExec_t eval_graph(Ctx &c, bool lambda_update, bool lipschitz, const float2 *ref_now,
-                                 const float2 *ref_prev, const float2 *grad_prev) {
+                                 const float2 *ref_prev, const float2 *grad_prev, int next_lambda, float2 *pos_out,
+                                 float2 *gtot_out) {
   for (const Ctx::EvalGraph &g : c.eval_graphs)
     if (g.lambda_update == lambda_update && g.lipschitz == lipschitz && g.ref_now == ref_now &&
-        g.ref_prev == ref_prev && g.grad_prev == grad_prev)
+        g.ref_prev == ref_prev && g.grad_prev == grad_prev && g.next_lambda == next_lambda &&
+        g.pos_out == pos_out && g.gtot_out == gtot_out)
       return g.exec;
   if (!c.graphs_ok) return nullptr;
   hipGraph_t graph = nullptr;
@@ -570,7 +576,8 @@ static hipGraphExec_t eval_graph(Ctx &c, bool lambda_update, bool lipschitz, con
     c.graphs_ok = false;
     return nullptr;
   }
-  enqueue_evaluation(c, lambda_update, lipschitz, ref_now, ref_prev, grad_prev, nullptr);
+  enqueue_evaluation(c, lambda_update, lipschitz, ref_now, ref_prev, grad_prev, next_lambda, pos_out, gtot_out,
+                     nullptr);
   hipError_t e = hipStreamEndCapture(c.stream, &graph);
   if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   if (graph) (void)hipGraphDestroy(graph);
@@ -579,7 +586,8 @@ static hipGraphExec_t eval_graph(Ctx &c, bool lambda_update, bool lipschitz, con
     (void)hipGetLastError();
     return nullptr;
   }
-  c.eval_graphs.push_back({lambda_update, lipschitz, ref_now, ref_prev, grad_prev, exec});
+  c.eval_graphs.push_back({lambda_update, lipschitz, ref_now, ref_prev, grad_prev, next_lambda, pos_out, gtot_out,
+                           exec});
   return exec;
 }
 
@@ -587,7 +595,8 @@ static hipGraphExec_t eval_graph(Ctx &c, bool lambda_update, bool lipschitz, con
 // replayed from its graph, or launched kernel by kernel when its phases are
 // timed.
 void evaluate(Ctx &c, bool lambda_update, bool lipschitz, const float2 *ref_now, const float2 *ref_prev,
-              const float2 *grad_prev, bool timed) {
+              const float2 *grad_prev, bool timed, int next_lambda = 0, float2 *pos_out = nullptr,
+              float2 *gtot_out = nullptr) {
   hipEvent_t *ev = nullptr;
   if (timed) {
     ev = c.ev;
@@ -596,14 +605,15 @@ void evaluate(Ctx &c, bool lambda_update, bool lipschitz, const float2 *ref_now,
     c.prof_used += kEvPerEval;
   }
   if (!ev) {
-    hipGraphExec_t g = eval_graph(c, lambda_update, lipschitz, ref_now, ref_prev, grad_prev);
+    hipGraphExec_t g = eval_graph(c, lambda_update, lipschitz, ref_now, ref_prev, grad_prev, next_lambda, pos_out,
+                                  gtot_out);
     if (g && hipGraphLaunch(g, c.stream) == hipSuccess) return;
   }
   // profiled: the device waits until the whole evaluation is enqueued, so the
   // phases run back to back as in a graph replay and the events time kernels,
   // not the host's submissions
   if (ev && !timed) launch_hold(c, c.prof_hold_ticks);
-  enqueue_evaluation(c, lambda_update, lipschitz, ref_now, ref_prev, grad_prev, ev);
+  enqueue_evaluation(c, lambda_update, lipschitz, ref_now, ref_prev, grad_prev, next_lambda, pos_out, gtot_out, ev);
 }
 
 // Phase times of one evaluation from its events: scatter, finalise, wiring
@@ -813,20 +823,28 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
       // combine-only re-evaluation at y_k with the updated lambda (the
       // positions were evaluated by the accepted trial or the caller)
       // (y_k and g_k are also copied for the trials: cur_ref, cur_grad)
+      // From the second iteration on, the accepted trial's combine already
+      // computed g_k at the updated lambda, y_k, and the Lipschitz sums of
+      // that pass (kPartialLipNext).
       if (k == 0 && !c.eval_valid) {
         evaluate(c, false, true, nullptr, c.last_ref, c.last_grad, false);
         launch_positions_to_ref(c, c.cur_ref);
         launch_copy_float2(c, c.cur_grad, c.gtot, c.O);
-      } else {
+        launch_step_scalar(c, 0);
+      } else if (k == 0) {
         launch_gather_combine(c, false, true, nullptr, c.last_ref, c.last_grad, c.cur_ref, c.cur_grad);
+        launch_step_scalar(c, 0);
+      } else {
+        launch_step_scalar(c, 0, kPartialLipNext);
       }
-      launch_step_scalar(c, 0);
       if (features) launch_ctrl_step(c, ctrl, learned ? 1 : 0);
       int tries = 0;
       bool stopped = false;
       while (true) {
         launch_nesterov_move_ex(c, c.cur_ref, c.cur_grad, c.cur_main, c.new_main, c.new_ref, nullptr, nullptr);
-        evaluate(c, false, true, c.new_ref, c.cur_ref, c.cur_grad, false);
+        // y_k, g_k of the previous iteration (last_ref, last_grad) are free:
+        // the trial's positions and its gradient at the staged lambda go there
+        evaluate(c, false, true, c.new_ref, c.cur_ref, c.cur_grad, false, learned ? 2 : 1, c.last_ref, c.last_grad);
         launch_accept(c);
         DevScalars h;
         if ((rc = read_scalars(c, h))) return rc;
@@ -844,13 +862,14 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
         if (h.accepted || tries >= o.max_backtracks) break;
       }
       if (stopped) break;
+      // cur_ref, cur_grad := the last trial's y and g at the staged lambda;
+      // last_ref, last_grad := y_k, g_k
       std::swap(c.last_ref, c.cur_ref);
       std::swap(c.last_grad, c.cur_grad);
       std::swap(c.cur_main, c.new_main);
-      // updatePenaltyFactor with the HPWL of the accepted evaluation, or the
-      // controller's multiplier
-      launch_reduce_wirelength(c, learned ? 2 : 1);
-      launch_done_check_ex(c, k + 1, dlog, cap, 1);
+      // updatePenaltyFactor with the HPWL of the last trial (or the
+      // controller's floor), as staged by that trial, then the stop check
+      launch_done_check_ex(c, k + 1, dlog, cap, 1, 1);
       ++k;
       continue;
     }

@@ -134,6 +134,10 @@ struct DevScalars {
   // running mean of log(reference multiplier), weight 1/16 per update: how
   // fast the reference rule itself would grow lambda (a controller feature)
   double ref_mult_ema;
+  // updatePenaltyFactor staged by a backtracking trial's evaluation: lambda
+  // and the reference multiplier it would take if the trial is the
+  // iteration's last; the stop check applies them
+  double lambda_next, ref_mult_next;
   float feat[EP_CTRL_FEATURES];
   int backtracks_mark, trial, fallbacks, ctrl_active;
   int bb_short;                       // iterations that took the short Barzilai-Borwein step
@@ -154,7 +158,8 @@ constexpr int kPartialRho = 7 * kMaxPartials;      // sum of rho per block
 constexpr int kPartialNorm = 8 * kMaxPartials;     // |gwl|^2, |gd|^2 pairs
 constexpr int kPartialRhoMax = 10 * kMaxPartials;  // max of rho per block
 constexpr int kPartialSY = 11 * kMaxPartials;      // (y - y_prev).(g - g_prev), stride 2
-constexpr int kPartialTotal = 13 * kMaxPartials;
+constexpr int kPartialLipNext = 13 * kMaxPartials; // num, den pairs at the staged lambda
+constexpr int kPartialTotal = 15 * kMaxPartials;
 
 struct Poisson;  // poisson.hip
 
@@ -205,6 +210,8 @@ struct Ctx {
   struct EvalGraph {
     bool lambda_update, lipschitz;
     const float2 *ref_now, *ref_prev, *grad_prev;
+    int next_lambda;
+    float2 *pos_out, *gtot_out;
     hipGraphExec_t exec;
   };
   std::vector<EvalGraph> eval_graphs;
@@ -308,13 +315,17 @@ void launch_gather_combine(Ctx &c, bool recompute_gather, bool lipschitz, const 
 void launch_field_gather(Ctx &c);
 void launch_node_wl(Ctx &c, hipStream_t st);
 void launch_combine(Ctx &c, bool recompute, bool lipschitz, const float2 *ref_now, const float2 *ref_prev,
-                    const float2 *grad_prev, float2 *pos_out = nullptr, float2 *gtot_out = nullptr);
+                    const float2 *grad_prev, float2 *pos_out = nullptr, float2 *gtot_out = nullptr,
+                    bool at_next_lambda = false);
 bool tile_sort_create(Ctx &c, std::string &err);
 void tile_sort_destroy(Ctx &c);
 // ---- wirelength.hip ----
 // st: the stream to launch on (nullptr: the context's stream)
 void launch_wirelength(Ctx &c, bool gradient, hipStream_t st = nullptr);
-// with_value: the wiring pass computed WA values (its gradient variant)
+// with_value: the wiring pass computed WA values (its gradient variant).
+// lambda_update: 0 none, 1 updatePenaltyFactor, 2 the same with the
+// controller's floor; 3 and 4 stage 1 and 2 in lambda_next / ref_mult_next
+// without applying them.
 void launch_reduce_wirelength(Ctx &c, int lambda_update, hipStream_t st = nullptr, bool with_value = true);
 // ---- poisson.hip ----
 bool poisson_create(Ctx &c, std::string &err);
@@ -324,14 +335,14 @@ void launch_poisson(Ctx &c, bool with_phi, hipEvent_t *gemm_ev = nullptr);
 // ---- nesterov.hip ----
 void launch_positions_to_ref(Ctx &c, float2 *dst);
 // bb: the Barzilai-Borwein step (nesterov.hpp:231-302) instead of 1/L
-void launch_step_scalar(Ctx &c, int bb);
+void launch_step_scalar(Ctx &c, int bb, int lip_slot = kPartialLip);
 void launch_accept(Ctx &c);
 void launch_lambda_init(Ctx &c);
 void launch_set_positions(Ctx &c, const float2 *xy_stage, const int32_t *stage_idx, int64_t n);
 void launch_copy_float2(Ctx &c, float2 *dst, const float2 *src, int64_t n);
 void launch_publish_scalars(Ctx &c, unsigned int seq);
 void launch_hold(Ctx &c, unsigned long long ticks);
-void launch_done_check_ex(Ctx &c, int next_iter, double *log, int log_cap, int set_iter);
+void launch_done_check_ex(Ctx &c, int next_iter, double *log, int log_cap, int set_iter, int apply_next = 0);
 void launch_nesterov_move_ex(Ctx &c, const float2 *ref, const float2 *grad, const float2 *main_in,
                              float2 *main_out, float2 *ref_unclamped_out, float2 *save_ref, float2 *save_grad);
 int poisson_ld(const Ctx &c);

@@ -47,9 +47,9 @@ __global__ void copy_float2_kernel(int64_t n, float2 *dst, const float2 *src) {
 // positive, else min(|s| / |y|, 1/L), s = y_k - y_{k-1}, y = g_k - g_{k-1}.
 // Also the Nesterov coefficient a_{k+1} and the controller's gradient norms.
 __global__ void __launch_bounds__(kFinish) step_scalar_kernel(int nblocks, double *partials, DevScalars *sc,
-                                                              int bb) {
+                                                              int bb, int lip_slot) {
   if (sc->done) return;
-  const double *const src[5] = {partials + kPartialLip, partials + kPartialLip + 1, partials + kPartialNorm,
+  const double *const src[5] = {partials + lip_slot, partials + lip_slot + 1, partials + kPartialNorm,
                                 partials + kPartialNorm + 1, partials + kPartialSY};
   double sum[5];
   block_sum_partials_strided_n(src, nblocks, 2, sum);
@@ -138,8 +138,16 @@ __global__ void __launch_bounds__(kBlock) nesterov_move_kernel(
 // End of an iteration: commit a_{k+1}, advance the counter, evaluate the
 // stop condition for the next iteration (nesterov.hpp:86-149) with the tau
 // of the last evaluation, append a log row.
-__global__ void done_check_kernel(DevScalars *sc, int next_iter, double *log, int log_cap, int set_iter) {
+__global__ void done_check_kernel(DevScalars *sc, int next_iter, double *log, int log_cap, int set_iter,
+                                  int apply_next) {
   if (sc->done) return;
+  // updatePenaltyFactor as the iteration's last trial staged it
+  // (reduce_wl_kernel, lambda_update 3 and 4)
+  if (apply_next) {
+    sc->ref_mult_ema += (::log(sc->ref_mult_next) - sc->ref_mult_ema) * (1.0 / 16.0);
+    sc->lambda = sc->lambda_next;
+    sc->last_hpwl = sc->hpwl;
+  }
   int k = next_iter - 1;
   if (log && k >= 0 && k < log_cap) {
     double *r = log + (size_t)k * EP_LOG_COLS;
@@ -193,9 +201,9 @@ void launch_copy_float2(Ctx &c, float2 *dst, const float2 *src, int64_t n) {
   hipLaunchKernelGGL(copy_float2_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, c.stream, n, dst, src);
 }
 
-void launch_step_scalar(Ctx &c, int bb) {
+void launch_step_scalar(Ctx &c, int bb, int lip_slot) {
   hipLaunchKernelGGL(step_scalar_kernel, dim3(1), dim3(kFinish), 0, c.stream, object_blocks(c.O), c.partials,
-                     c.sc, bb);
+                     c.sc, bb, lip_slot);
 }
 
 void launch_accept(Ctx &c) {
@@ -209,8 +217,9 @@ void launch_nesterov_move_ex(Ctx &c, const float2 *ref, const float2 *grad, cons
                      core_of(c), c.sc);
 }
 
-void launch_done_check_ex(Ctx &c, int next_iter, double *log, int log_cap, int set_iter) {
-  hipLaunchKernelGGL(done_check_kernel, dim3(1), dim3(1), 0, c.stream, c.sc, next_iter, log, log_cap, set_iter);
+void launch_done_check_ex(Ctx &c, int next_iter, double *log, int log_cap, int set_iter, int apply_next) {
+  hipLaunchKernelGGL(done_check_kernel, dim3(1), dim3(1), 0, c.stream, c.sc, next_iter, log, log_cap, set_iter,
+                     apply_next);
 }
 
 // The scalars to pinned coherent host memory, then the sequence number with

@@ -392,12 +392,18 @@ __global__ void __launch_bounds__(kFinish) reduce_wl_kernel(int nslots, double *
       double delta = hp - sc->last_hpwl;
       double mult = delta < 0.0 ? 1.05 : pow(1.05, -delta / 350000.0 + 1.0);
       mult = fmin(1.05, fmax(0.95, mult));
-      sc->ref_mult_ema += (log(mult) - sc->ref_mult_ema) * (1.0 / 16.0);
+      const double ref_mult = mult;
       // the controller's floor of this iteration (0.95: none), kept inside
       // the reference's own multiplier range
-      if (lambda_update == 2) mult = fmin(1.05, fmax(mult, sc->lam_floor));
-      sc->lambda *= mult;
-      sc->last_hpwl = hp;
+      if (lambda_update == 2 || lambda_update == 4) mult = fmin(1.05, fmax(mult, sc->lam_floor));
+      if (lambda_update <= 2) {
+        sc->ref_mult_ema += (log(ref_mult) - sc->ref_mult_ema) * (1.0 / 16.0);
+        sc->lambda *= mult;
+        sc->last_hpwl = hp;
+      } else {
+        sc->lambda_next = sc->lambda * mult;
+        sc->ref_mult_next = ref_mult;
+      }
     }
   }
 }
