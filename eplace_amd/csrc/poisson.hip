@@ -461,6 +461,17 @@ __global__ void __launch_bounds__(kGroups * kGroupThreads) pgemm_kernel(PGemm p)
       acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
       odd[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
     }
+  // the spectral divisor's wave numbers of this thread's outputs, loaded
+  // before the k loop so that their latency is not the epilogue's
+  float wx2[2][4] = {}, wy2[2] = {};
+  if (p.epi == 1 && grp == 0) {
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) wx2[mi][r] = p.wx2p[p.row_off[z] + roww + mi * 16 + 4 * (lane >> 4) + r];
+#pragma unroll
+    for (int ni = 0; ni < 2; ++ni) wy2[ni] = p.wy2p[colw + ni * 16 + (lane & 15)];
+  }
   if (p.out_fold) {
     run_k<ST>(p, z, B, sb, sub, bscale, row0, col0, 0, p.khalf, s[grp], acc);
     run_k<ST>(p, z, B, sb, sub, bscale, row0, col0, p.khalf, p.K, s[grp], odd);
@@ -494,7 +505,7 @@ __global__ void __launch_bounds__(kGroups * kGroupThreads) pgemm_kernel(PGemm p)
             wmax = fmaxf(wmax, fmaxf(fabsf(v0), fabsf(v1)));
           }
         } else if (p.epi == 1) {
-          const float a = p.wx2p[p.row_off[z] + row], b = p.wy2p[col];
+          const float a = wx2[mi][r], b = wy2[ni];
           const bool dc = p.row_off[z] + row == 0 && col == 0;
           const float v = (a < 0.f || b < 0.f || dc) ? 0.f : e / (a + b);
           C[(size_t)row * p.ldc + col] = v;
