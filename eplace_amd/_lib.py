@@ -99,6 +99,21 @@ class EPLegalStats(C.Structure):
                 ("total_displacement", C.c_double), ("max_displacement", C.c_double)]
 
 
+EP_DP_MAX_PASSES = 8
+
+
+class EPDPParams(C.Structure):
+    _fields_ = [("site_step", C.c_double), ("ism_time_limit", C.c_double), ("passes", C.c_int32),
+                ("pad", C.c_int32)]
+
+
+class EPDPStats(C.Structure):
+    _fields_ = [("hpwl_before", C.c_double), ("hpwl_after", C.c_double), ("seconds", C.c_double),
+                ("hpwl_after_lr", C.c_double * EP_DP_MAX_PASSES), ("hpwl_after_ism", C.c_double * EP_DP_MAX_PASSES),
+                ("hpwl_after_gs", C.c_double * EP_DP_MAX_PASSES), ("lr_windows", C.c_int64),
+                ("ism_matchings", C.c_int64), ("gs_moves", C.c_int64)]
+
+
 class EPLegalCheck(C.Structure):
     _fields_ = [("cells", C.c_int64), ("overlaps", C.c_int64), ("off_row", C.c_int64), ("off_site", C.c_int64),
                 ("out_of_core", C.c_int64), ("in_blockage", C.c_int64), ("macro_overlaps", C.c_int64),
@@ -162,6 +177,7 @@ SYMBOLS = {
     "ep_host_last_error": ([], C.c_char_p),
     "ep_legalize_abacus": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "ep_check_legal": ([C.c_void_p, C.c_void_p, C.c_double, C.c_void_p], C.c_int),
+    "ep_detailed_place": ([C.c_void_p] * 5, C.c_int),
     "ep_qp_last_error": ([], C.c_char_p),
     "ep_qplace": ([C.c_void_p] * 5, C.c_int),
     "ep_qp_matrix": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p], C.c_int),

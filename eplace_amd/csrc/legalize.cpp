@@ -31,22 +31,13 @@
 #include <vector>
 
 #include "../../include/eplace_amd.h"
+#include "host_rows.h"
 
 namespace {
 
+using namespace ep_rows;
+
 thread_local std::string g_err;
-
-constexpr double kEps = 1.0e-15;  // global.h:39
-// global.h:345-367 take float arguments.
-inline bool f_greater(float a, float b) { return (double)(a - b) > kEps; }
-inline bool f_less(float a, float b) { return (double)(a - b) < -kEps; }
-inline bool f_equal(float a, float b) { return std::fabs((double)(a - b)) < kEps; }
-inline bool f_lessorequal(float a, float b) { return f_less(a, b) || f_equal(a, b); }
-inline bool f_greaterorequal(float a, float b) { return f_greater(a, b) || f_equal(a, b); }
-
-struct Interval {
-  float start, end;
-};
 
 struct Cluster {
   float x = 0, e = 0, w = 0, q = 0;
@@ -58,124 +49,6 @@ struct Subrow {
   double bottom = 0, height = 0, step = 1, width = 0;
   std::vector<Cluster> clusters;
 };
-
-struct Geometry {
-  int N = 0;
-  std::vector<float> llx, lly, w, h;
-  std::vector<uint8_t> macro;
-  float core_llx = 0, core_lly = 0, core_urx = 0, core_ury = 0;
-};
-
-Geometry geometry(const ep_design *d, const float *xy) {
-  Geometry g;
-  g.N = d->num_nodes;
-  g.llx.resize(g.N);
-  g.lly.resize(g.N);
-  g.w.assign(d->node_w, d->node_w + g.N);
-  g.h.assign(d->node_h, d->node_h + g.N);
-  g.macro.resize(g.N);
-  const float rowh = (float)d->common_row_height;
-  for (int i = 0; i < g.N; ++i) {
-    const float cx = xy ? xy[2 * i] : d->node_x[i], cy = xy ? xy[2 * i + 1] : d->node_y[i];
-    g.llx[i] = cx - (float)0.5 * g.w[i];  // Module::setCenter_2D (objects.cpp:530-539)
-    g.lly[i] = cy - (float)0.5 * g.h[i];
-    g.macro[i] = f_greater(g.h[i], rowh);  // PlaceDB::addNode (placedb.cpp:102-111)
-  }
-  const int R = d->num_rows;
-  g.core_lly = (float)d->row_bottom[0];
-  g.core_ury = (float)(d->row_bottom[R - 1] + d->row_height[R - 1]);
-  g.core_llx = (float)d->row_x0[0];
-  g.core_urx = (float)d->row_x1[0];
-  for (int r = 0; r < R; ++r) {
-    g.core_llx = std::min(g.core_llx, (float)d->row_x0[r]);
-    g.core_urx = std::max(g.core_urx, (float)d->row_x1[r]);
-  }
-  return g;
-}
-
-struct Rect {
-  float llx, lly, urx, ury;
-};
-
-// Obstacles of removeBlockedSite (placedb.cpp:629-690): terminals that meet
-// the core, and every macro.
-std::vector<Rect> obstacles(const ep_design *d, const Geometry &g) {
-  std::vector<Rect> ob;
-  for (int t = 0; t < d->num_terminals; ++t) {
-    Rect r{d->term_x[t] - (float)0.5 * d->term_w[t], d->term_y[t] - (float)0.5 * d->term_h[t], 0, 0};
-    r.urx = r.llx + d->term_w[t];
-    r.ury = r.lly + d->term_h[t];
-    if (f_greater(g.core_ury, r.lly) && f_less(g.core_lly, r.ury) && f_greater(g.core_urx, r.llx) &&
-        f_less(g.core_llx, r.urx))
-      ob.push_back(r);
-  }
-  for (int i = 0; i < g.N; ++i)
-    if (g.macro[i]) ob.push_back(Rect{g.llx[i], g.lly[i], g.llx[i] + g.w[i], g.lly[i] + g.h[i]});
-  return ob;
-}
-
-// Row intervals after removing blocked sites (placedb.cpp:629-849).
-bool row_intervals(const ep_design *d, const Geometry &g, double step, std::vector<std::vector<Interval>> &iv) {
-  const int R = d->num_rows;
-  iv.assign(R, {});
-  for (int r = 0; r < R; ++r) iv[r].push_back(Interval{(float)d->row_x0[r], (float)d->row_x1[r]});
-  std::vector<Rect> ob = obstacles(d, g);
-  bool overlap = false;
-  std::sort(ob.begin(), ob.end(), [&](const Rect &a, const Rect &b) {
-    if (!f_equal(a.llx, b.llx)) return f_less(a.llx, b.llx);
-    if (!f_equal(a.lly, b.lly)) return f_less(a.lly, b.lly);
-    overlap = true;
-    return false;
-  });
-  if (overlap) {
-    g_err = "terminals or macros share a lower-left corner (they overlap)";
-    return false;
-  }
-  for (const Rect &o : ob) {
-    int rb = 0;
-    while (rb < R && !(d->row_bottom[rb] + d->row_height[rb] > o.lly)) ++rb;
-    if (rb == R) continue;  // above every row
-    int re = rb;
-    while (re < R && !(d->row_bottom[re] + d->row_height[re] >= o.ury)) ++re;
-    if (re == R) --re;
-    for (int r = rb; r <= re; ++r) {
-      std::vector<Interval> &v = iv[r];
-      for (int i = 0; i < (int)v.size(); ++i) {
-        const Interval t = v[i];
-        if (t.start >= o.urx || t.end <= o.llx) continue;
-        if (t.start >= o.llx && t.end <= o.urx) {
-          v.erase(v.begin() + i);  // the next interval shifts into i and is skipped, as in the reference
-        } else if (t.end > o.urx && t.start >= o.llx) {
-          v[i].start = o.urx;
-        } else if (t.start < o.llx && t.end <= o.urx) {
-          v[i].end = o.llx;
-        } else if (t.start < o.llx && t.end > o.urx) {
-          v[i].end = o.llx;
-          v.insert(v.begin() + i + 1, Interval{o.urx, t.end});
-        }
-      }
-    }
-  }
-  for (auto &v : iv) {
-    for (auto it = v.begin(); it != v.end();) {
-      const double width = (double)(it->end - it->start);
-      if (f_less((float)width, (float)step)) {
-        it = v.erase(it);
-        continue;
-      }
-      const double left = std::ceil((it->start - g.core_llx) / step) * step + g.core_llx;
-      const double right = std::floor((it->end - g.core_llx) / step) * step + g.core_llx;
-      if (f_greater((float)(right - left), 0.f)) {
-        it->start = (float)left;
-        it->end = (float)right;
-        ++it;
-      } else {
-        it = v.erase(it);
-      }
-    }
-  }
-  return true;
-}
 
 struct Abacus {
   const Geometry &g;
@@ -289,6 +162,8 @@ struct Abacus {
 
 }  // namespace
 
+std::string &ep_rows::host_error() { return g_err; }
+
 extern "C" {
 
 const char *ep_host_last_error(void) { return g_err.c_str(); }
@@ -300,7 +175,7 @@ int ep_legalize_abacus(const ep_design *d, const float *node_xy, const ep_legal_
   if (!(p->site_step > 0) || (p->mode != 0 && p->mode != 1)) return g_err = "bad site_step or mode", EP_ERR_ARG;
   Geometry g = geometry(d, node_xy);
   std::vector<std::vector<Interval>> iv;
-  if (!row_intervals(d, g, p->site_step, iv)) return EP_ERR_ARG;
+  if (!row_intervals(d, g, p->site_step, iv, g_err)) return EP_ERR_ARG;
   Abacus a{g, p->mode, {}};
   for (int r = 0; r < d->num_rows; ++r)
     for (const Interval &t : iv[r]) {

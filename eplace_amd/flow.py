@@ -7,6 +7,8 @@ the library:
     FILLERONLY    switch2FillerOnly + EplaceNesterovOpt (after mLG, unless cgp=False, -nocGP)
     cGP           switch2cGP + EplaceNesterovOpt        (after FILLERONLY)
     legalisation  AbacusLegalizer                       (legal=True, -internalLegal)
+    detailed      DetailedPlacer::detailedPlacement     (dp=True after legal=True, -internalDP,
+                                                         ePlace_main.cpp:271-283)
 
 The design's node centres are updated with the result, as the reference
 writes its database.
@@ -18,17 +20,18 @@ import time
 import numpy as np
 
 from .design import Design
-from .legalize import SAMacroLegalizer, check_legal, legalize_abacus
+from .legalize import SAMacroLegalizer, check_legal, detailed_place, legalize_abacus
 from .placer import EPlacer2D, EplaceNesterovOpt
 from .qplace import quadratic_placement
 
 
 def place(d: Design, target_density: float = 1.0, target_overflow: float = 0.10, *, qp: bool = True,
           qp_iterations: int = 20, mgp: bool = True, mlg: bool = True, cgp: bool = True, legal: bool = False,
-          method: str = "vanilla", controller=None, bins=0, wirelength: str = "WA", poisson_mode: int = 0,
+          dp: bool = False, method: str = "vanilla", controller=None, bins=0, wirelength: str = "WA", poisson_mode: int = 0,
           device: int = 0, verbose: bool = False) -> dict:
     """Runs the flow on d and returns per-stage results; d.node_x/node_y hold
-    the final node centres (the legal ones when legal=True).  method
+    the final node centres (the legal ones when legal=True, after detailed
+    placement when dp=True as well).  method
     "learned" takes its step / lambda controller from `controller`."""
     say = print if verbose else (lambda *a, **k: None)
     out: dict = {}
@@ -63,6 +66,11 @@ def place(d: Design, target_density: float = 1.0, target_overflow: float = 0.10,
         lxy, lst = legalize_abacus(d, xy, mode=0)
         out["legal"] = dict(lst, check=check_legal(d, lxy))
         xy = lxy
+        if dp:
+            dxy, dst = detailed_place(d, xy)
+            out["dp"] = dict(dst, check=check_legal(d, dxy))
+            say(f"DP: HPWL {dst['hpwl_before']:.6g} -> {dst['hpwl_after']:.6g}")
+            xy = dxy
     d.node_x[:] = xy[:, 0]
     d.node_y[:] = xy[:, 1]
     out["positions"] = xy

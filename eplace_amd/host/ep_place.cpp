@@ -3,7 +3,7 @@
 //
 //   ep_place -aux <design.aux> [-targetDensity 1.0] [-targetOverflow 0.10]
 //            [-loadpl <file.pl>] [-noQP] [-IPiteCount <n>] [-bktrk] [-bb] [-LSE]
-//            [-bins <n>] [-nomGP] [-internalLegal] [-outputPath <dir>]
+//            [-bins <n>] [-nomGP] [-internalLegal] [-internalDP] [-outputPath <dir>]
 //            [-dumpDesign <file>]
 //
 // Reads the Bookshelf design, runs QP (QPPlacer::quadraticPlacement from the
@@ -16,6 +16,9 @@
 // -internalLegal runs the Abacus legaliser on the result (reference
 // semantics, ePlace_main.cpp:248-259), checks it and writes
 // <name>-legal.pl beside the global placement.
+// -internalDP then runs the detailed placer on the legal result
+// (DetailedPlacer::detailedPlacement, ePlace_main.cpp:271-283), checks it
+// and writes <name>-eDP.pl.
 // -dumpDesign writes the parsed arrays (ep_design field order, raw
 // little-endian) to a file and exits before any device work.
 #include <sys/stat.h>
@@ -36,7 +39,7 @@ namespace {
 struct Args {
   std::string aux, loadpl, out = ".", dump;
   float td = 1.0f, tof = 0.10f;
-  bool bktrk = false, bb = false, lse = false, no_mgp = false, legal = false, no_qp = false, no_mlg = false, no_cgp = false;
+  bool bktrk = false, bb = false, lse = false, no_mgp = false, legal = false, dp = false, no_qp = false, no_mlg = false, no_cgp = false;
   int bins = 0, qp_iters = 20;
   // -bb wins over -bktrk, as in EplaceNesterovOpt::opt_step (nesterov.hpp:157-165)
   int method() const { return bb ? STEP_BB : bktrk ? STEP_BKTRK : STEP_VANILLA; }
@@ -46,7 +49,7 @@ int usage(const char *prog) {
   fprintf(stderr,
           "usage: %s -aux <design.aux> [-targetDensity d] [-targetOverflow t] [-loadpl f.pl]\n"
           "          [-noQP] [-IPiteCount n] [-bktrk] [-bb] [-LSE] [-bins n] [-nomGP] [-nomLG] [-nocGP]\n"
-          "          [-internalLegal]\n"
+          "          [-internalLegal] [-internalDP]\n"
           "          [-outputPath dir] [-dumpDesign file]\n",
           prog);
   return 2;
@@ -88,6 +91,8 @@ bool parse(int argc, char **argv, Args &a) {
       a.no_mgp = true;
     } else if (k == "-internalLegal") {
       a.legal = true;
+    } else if (k == "-internalDP") {
+      a.dp = true;
     } else if (k == "-IPiteCount") {
       if (!val(v)) return false;
       a.qp_iters = atoi(v.c_str());
@@ -238,6 +243,28 @@ int main(int argc, char **argv) {
       write_pl(d, lxy, lpl);
       printf("wrote %s\n", lpl.c_str());
       if (!ok) return 3;
+      if (a.dp) {
+        std::vector<float> dxy(xy.size());
+        ep_dp_params dpp{};
+        dpp.site_step = d.site_width;
+        dpp.passes = 2;
+        ep_dp_stats ds{};
+        if (ep_detailed_place(&view, lxy.data(), &dpp, dxy.data(), &ds) != EP_OK)
+          throw std::runtime_error(std::string("detailed placement: ") + ep_host_last_error());
+        ep_legal_check dchk{};
+        if (ep_check_legal(&view, dxy.data(), d.site_width, &dchk) != EP_OK)
+          throw std::runtime_error(std::string("legality check: ") + ep_host_last_error());
+        const bool dok = dchk.overlaps == 0 && dchk.off_row == 0 && dchk.off_site == 0 && dchk.out_of_core == 0 &&
+                         dchk.in_blockage == 0 && dchk.macro_overlaps == 0;
+        printf("HPWL before detailed placement: %.0f\nHPWL after detailed placement: %.0f (%.2fs); %s\n",
+               ds.hpwl_before, ds.hpwl_after, ds.seconds, dok ? "legal" : "NOT legal");
+        std::string dpl = dir + d.name + "-eDP.pl";
+        write_pl(d, dxy, dpl);
+        printf("wrote %s\n", dpl.c_str());
+        if (!dok) return 3;
+      }
+    } else if (a.dp) {
+      printf("-internalDP runs after -internalLegal only; skipped\n");
     }
   } catch (const std::exception &e) {
     fprintf(stderr, "ep_place: %s\n", e.what());

@@ -8,6 +8,7 @@
 //   lines 141-168  EPlacer_2D + EplaceNesterovOpt + opt()     -> EPlacerAMD + opt()
 //   lines 174-186  SAMacroLegalizer(placedb)->legalization()  -> EPlacerAMD::macroLegalization
 //   lines 188-219  switch2FillerOnly / opt / switch2cGP / opt -> the same names on EPlacerAMD
+//   lines 271-283  DetailedPlacer(placedb)->detailedPlacement() -> detailedPlacementAMD
 //
 // Numbering: nodes are 0..N-1 in dbNodes order, terminals N..N+T-1 in
 // dbTerminals order (the parser gives each list its own idx, parser.cpp:412-433).
@@ -103,6 +104,23 @@ inline ep_qp_stats quadraticPlacementAMD(PlaceDB *db, int maxIterations = 20, in
     ep_qp_stats st{};
     if (ep_qplace(&a.d, &q, xy.data(), log.data(), &st) != EP_OK)
         throw std::runtime_error(ep_qp_last_error());
+    writeNodeCenters(db, xy);
+    return st;
+}
+
+// DetailedPlacer(db)->detailedPlacement() (DetailedPlacement/detailed.cpp:74-85)
+// in the library on the database's legal node positions (after the
+// reference's AbacusLegalizer, as -internalLegal -internalDP run them); the
+// nodes move to the result.
+inline ep_dp_stats detailedPlacementAMD(PlaceDB *db, int passes = 2) {
+    DesignArrays a(db);
+    ep_dp_params p{};
+    p.site_step = db->dbSiteRows.front().step;
+    p.passes = passes;
+    std::vector<float> xy(2 * (size_t)a.d.num_nodes);
+    ep_dp_stats st{};
+    if (ep_detailed_place(&a.d, nullptr, &p, xy.data(), &st) != EP_OK)
+        throw std::runtime_error(ep_host_last_error());
     writeNodeCenters(db, xy);
     return st;
 }

@@ -51,6 +51,31 @@ def check_legal(d: Design, xy=None, site_step: float | None = None) -> dict:
     return out
 
 
+def detailed_place(d: Design, xy=None, passes: int = 2, site_step: float | None = None,
+                   ism_time_limit: float = 0.0):
+    """DetailedPlacer::detailedPlacement (DetailedPlacement/detailed.cpp:74-85)
+    on legal node centres: `passes` rounds of local reordering, independent-set
+    matching and global swap.  Returns the centres [N, 2] and stats (HPWL
+    before, after and after each move of each round)."""
+    lib = L.load()
+    sd, keep = design_struct(d)
+    inp = _centres(d, xy)
+    out = np.zeros_like(inp)
+    p = L.EPDPParams(site_step=float(site_step or d.site_width), ism_time_limit=float(ism_time_limit),
+                     passes=int(passes))
+    st = L.EPDPStats()
+    rc = lib.ep_detailed_place(C.byref(sd), inp.ctypes.data, C.byref(p), out.ctypes.data, C.byref(st))
+    if rc != L.EP_OK:
+        raise L.EPError(rc, lib.ep_host_last_error().decode())
+    del keep
+    n = min(int(passes), L.EP_DP_MAX_PASSES)
+    stats = {k: getattr(st, k) for k in ("hpwl_before", "hpwl_after", "seconds", "lr_windows", "ism_matchings",
+                                         "gs_moves")}
+    for k in ("hpwl_after_lr", "hpwl_after_ism", "hpwl_after_gs"):
+        stats[k] = list(getattr(st, k))[:n]
+    return out, stats
+
+
 def legalize_macros(d: Design, xy=None, target_density: float = 1.0, rand_seed: int = 1, rand_skip: int = 0,
                     max_outer: int = 0, max_inner: int = 0):
     """SAMacroLegalizer(db).legalization() on node centres xy (default: the

@@ -379,6 +379,29 @@ int ep_legalize_abacus(const ep_design *design, const float *node_xy, const ep_l
  * checker): row, site, core, blockage and overlap checks of a placement. */
 int ep_check_legal(const ep_design *design, const float *node_xy, double site_step, ep_legal_check *out);
 
+/* Detailed placement (host, no device work): DetailedPlacer::detailedPlacement
+ * (DetailedPlacement/detailed.cpp:74-85, ePlace_main.cpp:271-283 with
+ * -internalDP) on a legal placement: `passes` rounds of local reordering,
+ * independent-set matching and global swap.  node_xy: [num_nodes][2] legal
+ * centres (NULL: the design's); out_xy receives the centres after it (macros
+ * unchanged); errors are described by ep_host_last_error(). */
+#define EP_DP_MAX_PASSES 8
+typedef struct ep_dp_params {
+  double site_step;      /* site width of the rows (.scl Sitewidth) */
+  double ism_time_limit; /* seconds for a round's ISM sweeps; <= 0: the reference's 28800 (detailed.cpp:118) */
+  int32_t passes;        /* rounds, 2 (detailed.cpp:77) */
+  int32_t pad;
+} ep_dp_params;
+
+typedef struct ep_dp_stats {
+  double hpwl_before, hpwl_after, seconds;
+  double hpwl_after_lr[EP_DP_MAX_PASSES], hpwl_after_ism[EP_DP_MAX_PASSES], hpwl_after_gs[EP_DP_MAX_PASSES];
+  int64_t lr_windows, ism_matchings, gs_moves;
+} ep_dp_stats;
+
+int ep_detailed_place(const ep_design *design, const float *node_xy, const ep_dp_params *params, float *out_xy,
+                      ep_dp_stats *stats);
+
 /* ---------------------------------------------------------------------------
  * Learned step / lambda controller.  No counterpart in the reference: an MLP
  * EP_CTRL_FEATURES -> EP_CTRL_HIDDEN -> EP_CTRL_HIDDEN -> EP_CTRL_OUTPUTS

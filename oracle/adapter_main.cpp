@@ -12,10 +12,17 @@
 //
 //   adapter_main -aux F.aux -targetDensity TD [-noQP] [-loadpl F.pl]
 //                [-IPiteCount K] [-bktrk | -bb | -learned PARAMS.f32] [-LSE]
-//                [-mixed] -out DIR
+//                [-mixed] [-noGP] [-internalDP] -out DIR
 //
 // DIR/scalars.txt gets "key value" lines; DIR/node_centers.f32 the node
 // centres (dbNodes order, x/y float pairs) as the PlaceDB holds them at the end.
+// -noGP skips QP and the global placement (no device work).  -internalDP
+// then legalises with the reference's AbacusLegalizer and runs
+// detailedPlacementAMD, and, from the same legal positions, the reference's
+// own DetailedPlacer, and records both results and their largest difference
+// (the database keeps the library's).
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -26,6 +33,8 @@
 #include <vector>
 
 #include "arghandler.h"
+#include "detailed.h"
+#include "legalizer.h"
 #include "parser.h"
 #include "placedb.h"
 #include "eplace_amd_adapter.h"
@@ -79,67 +88,99 @@ int run(int argc, char **argv) {
   if (!pl.empty()) parser.ReadPLFile(pl, *db, false);
   scalar("hpwl_loaded", db->calcHPWL());
 
-  // ePlace_main.cpp:121-126
-  if (!arg_flag(argc, argv, "noQP")) {
-    const int k = atoi(arg_str(argc, argv, "IPiteCount", "20").c_str());
-    ep_qp_stats q = quadraticPlacementAMD(db, k);
-    scalar("qp_iterations", q.iterations);
-    scalar("qp_hpwl", db->calcHPWL());
-  }
-
-  // ePlace_main.cpp:141-168
-  const float td = (float)atof(arg_str(argc, argv, "targetDensity", "1.0").c_str());
-  scalar("hpwl_before_gp", db->calcHPWL());
-  EPlacerAMD gp(db, td, arg_flag(argc, argv, "LSE"));
-  // the library's HPWL of the same positions (penaltyFactorInitilization's
-  // lastHPWL): equal to the PlaceDB's only if the design arrays number the
-  // pins' modules as the library expects
-  scalar("ep_hpwl_initial", gp.stats().last_hpwl);
-  Step step = Step::Vanilla;
-  std::vector<float> ctrl;
-  if (arg_flag(argc, argv, "bktrk")) step = Step::Backtracking;
-  if (arg_flag(argc, argv, "bb")) step = Step::BarzilaiBorwein;
-  const std::string ctrlPath = arg_str(argc, argv, "learned", "");
-  if (!ctrlPath.empty()) {
-    std::ifstream f(ctrlPath, std::ios::binary);
-    ctrl.resize(EP_CTRL_PARAMS);
-    if (!f.read(reinterpret_cast<char *>(ctrl.data()), ctrl.size() * sizeof(float))) {
-      fprintf(stderr, "adapter_main: %s does not hold %d floats\n", ctrlPath.c_str(), (int)EP_CTRL_PARAMS);
-      return 2;
+  if (!arg_flag(argc, argv, "noGP")) {
+    // ePlace_main.cpp:121-126
+    if (!arg_flag(argc, argv, "noQP")) {
+      const int k = atoi(arg_str(argc, argv, "IPiteCount", "20").c_str());
+      ep_qp_stats q = quadraticPlacementAMD(db, k);
+      scalar("qp_iterations", q.iterations);
+      scalar("qp_hpwl", db->calcHPWL());
     }
-    gp.setController(ctrl.data());
-    step = Step::Learned;
-  }
-  ep_run_stats m = gp.opt(step);
-  printf("mGP finished! Final HPWL: %d (%d iterations)\n", int(db->calcHPWL()), m.iterations);
-  run_stats("mgp", m);
-  scalar("mgp_hpwl", db->calcHPWL());
 
-  // ePlace_main.cpp:174-219 (macro designs only)
-  if (arg_flag(argc, argv, "mixed") && db->dbMacroCount > 0) {
-    ep_mlg_stats l = gp.macroLegalization(td);
-    scalar("mlg_final_overlap", l.final_overlap);
-    scalar("mlg_hpwl", db->calcHPWL());
-    std::vector<POS_3D> macroAt;
-    for (Module *n : db->dbNodes)
-      if (n->isMacro) macroAt.push_back(n->getCenter());
-    gp.switch2FillerOnly();
-    run_stats("fo", gp.opt(step));
-    gp.switch2cGP();
-    ep_run_stats c = gp.opt(step);
-    run_stats("cgp", c);
-    scalar("cgp_hpwl", db->calcHPWL());
-    // FILLERONLY moves fillers only and cGP keeps the macros (eplace.cpp:887-904)
-    int moved = 0;
-    size_t k = 0;
-    for (Module *n : db->dbNodes)
-      if (n->isMacro) {
-        moved += n->getCenter().x != macroAt[k].x || n->getCenter().y != macroAt[k].y;
-        ++k;
+    // ePlace_main.cpp:141-168
+    const float td = (float)atof(arg_str(argc, argv, "targetDensity", "1.0").c_str());
+    scalar("hpwl_before_gp", db->calcHPWL());
+    EPlacerAMD gp(db, td, arg_flag(argc, argv, "LSE"));
+    // the library's HPWL of the same positions (penaltyFactorInitilization's
+    // lastHPWL): equal to the PlaceDB's only if the design arrays number the
+    // pins' modules as the library expects
+    scalar("ep_hpwl_initial", gp.stats().last_hpwl);
+    Step step = Step::Vanilla;
+    std::vector<float> ctrl;
+    if (arg_flag(argc, argv, "bktrk")) step = Step::Backtracking;
+    if (arg_flag(argc, argv, "bb")) step = Step::BarzilaiBorwein;
+    const std::string ctrlPath = arg_str(argc, argv, "learned", "");
+    if (!ctrlPath.empty()) {
+      std::ifstream f(ctrlPath, std::ios::binary);
+      ctrl.resize(EP_CTRL_PARAMS);
+      if (!f.read(reinterpret_cast<char *>(ctrl.data()), ctrl.size() * sizeof(float))) {
+        fprintf(stderr, "adapter_main: %s does not hold %d floats\n", ctrlPath.c_str(), (int)EP_CTRL_PARAMS);
+        return 2;
       }
-    scalar("macros", (double)k);
-    scalar("macros_moved_after_mlg", moved);
-    printf("cGP finished! Final HPWL: %d (%d iterations)\n", int(db->calcHPWL()), c.iterations);
+      gp.setController(ctrl.data());
+      step = Step::Learned;
+    }
+    ep_run_stats m = gp.opt(step);
+    printf("mGP finished! Final HPWL: %d (%d iterations)\n", int(db->calcHPWL()), m.iterations);
+    run_stats("mgp", m);
+    scalar("mgp_hpwl", db->calcHPWL());
+
+    // ePlace_main.cpp:174-219 (macro designs only)
+    if (arg_flag(argc, argv, "mixed") && db->dbMacroCount > 0) {
+      ep_mlg_stats l = gp.macroLegalization(td);
+      scalar("mlg_final_overlap", l.final_overlap);
+      scalar("mlg_hpwl", db->calcHPWL());
+      std::vector<POS_3D> macroAt;
+      for (Module *n : db->dbNodes)
+        if (n->isMacro) macroAt.push_back(n->getCenter());
+      gp.switch2FillerOnly();
+      run_stats("fo", gp.opt(step));
+      gp.switch2cGP();
+      ep_run_stats c = gp.opt(step);
+      run_stats("cgp", c);
+      scalar("cgp_hpwl", db->calcHPWL());
+      // FILLERONLY moves fillers only and cGP keeps the macros (eplace.cpp:887-904)
+      int moved = 0;
+      size_t k = 0;
+      for (Module *n : db->dbNodes)
+        if (n->isMacro) {
+          moved += n->getCenter().x != macroAt[k].x || n->getCenter().y != macroAt[k].y;
+          ++k;
+        }
+      scalar("macros", (double)k);
+      scalar("macros_moved_after_mlg", moved);
+      printf("cGP finished! Final HPWL: %d (%d iterations)\n", int(db->calcHPWL()), c.iterations);
+    }
+
+  }
+
+  // ePlace_main.cpp:249-283 with -internalLegal -internalDP
+  if (arg_flag(argc, argv, "internalDP")) {
+    AbacusLegalizer legal(db);
+    legal.legalization();
+    scalar("legal_hpwl", db->calcHPWL());
+    std::vector<POS_2D> at;
+    for (Module *n : db->dbNodes) at.push_back(n->getLL_2D());
+    // the reference's own detailed placer from the legal positions
+    DetailedPlacer ref(db);
+    ref.detailedPlacement();
+    scalar("dp_ref_hpwl", db->calcHPWL());
+    std::vector<POS_2D> refAt;
+    for (Module *n : db->dbNodes) refAt.push_back(n->getLL_2D());
+    for (size_t k = 0; k < at.size(); ++k) db->setModuleLocation_2D(db->dbNodes[k], at[k].x, at[k].y);
+    // the library's, through the adapter, from the same positions
+    ep_dp_stats d = detailedPlacementAMD(db);
+    scalar("dp_hpwl", db->calcHPWL());
+    scalar("dp_lib_hpwl_before", d.hpwl_before);
+    scalar("dp_lib_hpwl_after", d.hpwl_after);
+    double diff = 0;
+    for (size_t k = 0; k < at.size(); ++k) {
+      const POS_2D q = db->dbNodes[k]->getLL_2D();
+      diff = std::max(diff, (double)std::max(std::fabs(q.x - refAt[k].x), std::fabs(q.y - refAt[k].y)));
+    }
+    scalar("dp_max_diff_vs_ref", diff);
+    printf("HPWL before detailed placement: %d\nHPWL after detailed placement: %d\n", int(d.hpwl_before),
+           int(db->calcHPWL()));
   }
 
   std::vector<float> xy;

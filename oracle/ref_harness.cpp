@@ -26,6 +26,10 @@
 //   mlg  -aux F.aux -noQP [-centers C.f32] [-targetDensity TD] [-randSkip K] -out DIR
 //        (SAMacroLegalizer on the node centres after K rand() draws; the
 //        centres after it, its final cost terms and the next rand() value)
+//   dp   -aux F.aux -noQP [-centers C.f32] [-passes P] -out DIR
+//        (AbacusLegalizer on the centres, then DetailedPlacer's rounds as
+//        detailedPlacement runs them, the positions after each operator of
+//        each round: lr<i>, ism<i>, gs<i>, with their HPWL)
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -55,6 +59,7 @@
 // header only.
 #define private public
 #include "nesterov.hpp"
+#include "detailed.h"
 #undef private
 
 namespace {
@@ -430,6 +435,51 @@ int mode_legal(int argc, char **argv) {
   return 0;
 }
 
+// DetailedPlacer::detailedPlacement (detailed.cpp:74-85) one operator at a
+// time, so each operator's result is dumped.
+int mode_dp(int argc, char **argv) {
+  BookshelfParser parser;
+  PlaceDB *db = new PlaceDB();
+  load_design(argc, argv, parser, db);
+  dump_design(db);
+  std::string cin = arg_str(argc, argv, "centers", "");
+  if (!cin.empty()) {
+    std::vector<float> xy(db->dbNodes.size() * 2);
+    FILE *f = fopen(cin.c_str(), "rb");
+    if (!f || fread(xy.data(), 4, xy.size(), f) != xy.size()) {
+      fprintf(stderr, "bad centers file\n");
+      return 2;
+    }
+    fclose(f);
+    for (size_t i = 0; i < db->dbNodes.size(); ++i) db->setModuleCenter_2D(db->dbNodes[i], xy[2 * i], xy[2 * i + 1]);
+  }
+  AbacusLegalizer legal(db);
+  legal.legalization();
+  npy_f32("legal_positions", positions(db->dbNodes), 2);
+  scalar("legal_hpwl", db->calcHPWL());
+  const int passes = atoi(arg_str(argc, argv, "passes", "2").c_str());
+  DetailedPlacer dp(db);
+  struct timeval t0, t1;
+  gettimeofday(&t0, nullptr);
+  dp.initialization();
+  for (int i = 0; i < passes; ++i) {
+    const std::string k = std::to_string(i);
+    dp.runLocalReordering();
+    npy_f32("lr" + k, positions(db->dbNodes), 2);
+    scalar("lr" + k + "_hpwl", db->calcHPWL());
+    dp.runISM();
+    npy_f32("ism" + k, positions(db->dbNodes), 2);
+    scalar("ism" + k + "_hpwl", db->calcHPWL());
+    dp.runGlobalSwap();
+    npy_f32("gs" + k, positions(db->dbNodes), 2);
+    scalar("gs" + k + "_hpwl", db->calcHPWL());
+  }
+  gettimeofday(&t1, nullptr);
+  scalar("dp_hpwl", db->calcHPWL());
+  scalar("dp_seconds", (t1.tv_sec - t0.tv_sec) + 1e-6 * (t1.tv_usec - t0.tv_usec));
+  return 0;
+}
+
 void dump_coo(const std::string &tag, const SMatrix &A) {
   std::vector<int32_t> row, col;
   std::vector<float> val;
@@ -552,6 +602,7 @@ int main(int argc, char **argv) {
   if (mode == "legal") return mode_legal(argc, argv);
   if (mode == "qp") return mode_qp(argc, argv);
   if (mode == "mlg") return mode_mlg(argc, argv);
+  if (mode == "dp") return mode_dp(argc, argv);
   fprintf(stderr, "unknown mode %s\n", mode.c_str());
   return 2;
 }

@@ -1,7 +1,7 @@
 """Generate golden fixtures from the reference placer (run in the container
 that has the reference tree; the fixtures are committed, the reference is not).
 
-    python tests/golden/gen_golden.py [gp] [bb] [poisson] [legal] [qp] [mlg] [flow]   (default: all)
+    python tests/golden/gen_golden.py [gp] [bb] [poisson] [legal] [qp] [mlg] [flow] [dp]   (default: all)
 
 Builds oracle/_ref/ref_harness from the reference's own sources
 (oracle/Makefile.ref), runs it on the reference's smoke design and on seeded
@@ -32,7 +32,7 @@ KEEP = ["nodes_whxy", "nodes_macro", "terms_whxyll", "terms_ni", "pin_obj", "pin
         "it1_db_positions", "it2_db_positions", "trajectory", "final_positions", "legal_positions",
         "input_positions", "qp_ax_row", "qp_ax_col", "qp_ax_val", "qp_ay_row", "qp_ay_col", "qp_ay_val", "qp_bx",
         "qp_by", "qp_guess_x", "qp_guess_y", "qp_matrix_positions", "qp_positions", "mlg_positions", "mixed_mlg_positions", "mixed_fo_fillers",
-        "mixed_fo_positions", "mixed_cgp_positions"]
+        "mixed_fo_positions", "mixed_cgp_positions", "lr0", "ism0", "gs0", "lr1", "ism1", "gs1"]
 
 
 def run(args, out):
@@ -72,6 +72,8 @@ def main(parts):
         poisson_fixtures(tmp)
     if "legal" in parts:
         legal_fixtures(tmp)
+    if "dp" in parts:
+        dp_fixtures(tmp)
     if "qp" in parts:
         qp_fixtures(tmp)
     if "mlg" in parts:
@@ -158,6 +160,54 @@ def legal_fixtures(tmp):
         pack(name, tmp + "/out_" + name)
 
 
+def connect_all(d):
+    """The design with a two-pin net (centre pins) from every node without
+    pins to its index neighbour: GlobalSwapDP reads the optimal region of
+    every cell, which for a cell without nets indexes an empty vector
+    (placedb.cpp:1480-1533), so the reference's detailed placer needs every
+    cell connected."""
+    import dataclasses
+    n = d.num_nodes
+    cnt = np.bincount(d.pin_obj[d.pin_obj < n], minlength=n)
+    lone = np.flatnonzero(cnt == 0)
+    extra = np.stack([lone, np.where(lone + 1 < n, lone + 1, lone - 1)], 1).reshape(-1).astype(np.int32)
+    k = len(lone)
+    return dataclasses.replace(
+        d, net_start=np.concatenate([d.net_start, d.net_start[-1] + 2 * np.arange(1, k + 1)]).astype(np.int64),
+        pin_obj=np.concatenate([d.pin_obj, extra]).astype(np.int32),
+        pin_dx=np.concatenate([d.pin_dx, np.zeros(2 * k, np.float32)]),
+        pin_dy=np.concatenate([d.pin_dy, np.zeros(2 * k, np.float32)]))
+
+
+def dp_fixtures(tmp):
+    # 7. DetailedPlacer after AbacusLegalizer (ePlace_main.cpp:249-283 with
+    # -internalLegal -internalDP): the smoke design's GP result, the
+    # 2000-cell design's GP result (every cell connected), and a 20000-cell
+    # design from legalised random centres
+    d20 = connect_all(synthetic(num_cells=20000, num_nets=21000, seed=31, util=0.7, name="dp_rand20k"))
+    llx, lly, urx, ury = d20.core_region()
+    rng = np.random.default_rng(32)
+    xy20 = np.stack([rng.uniform(llx + d20.node_w / 2, urx - d20.node_w / 2),
+                     rng.uniform(lly + d20.node_h / 2, ury - d20.node_h / 2)], 1).astype(np.float32)
+    cases = {
+        "dp_smoke": (Golden_design("smoke_default"), np.load(os.path.join(HERE, "smoke_default.npz"))["final_positions"][:100]),
+        "dp_synth_c2": (connect_all(synthetic(num_cells=2000, num_nets=2100, seed=12, name="dp_synth_c2")),
+                        np.load(os.path.join(HERE, "synth_c2.npz"))["final_positions"][:2000]),
+        "dp_rand20k": (d20, xy20),
+    }
+    for name, (d, xy) in cases.items():
+        aux = d if isinstance(d, str) else write_bookshelf(d, os.path.join(tmp, name))
+        f = os.path.join(tmp, name + ".f32")
+        np.ascontiguousarray(xy, np.float32).tofile(f)
+        run(["dp", "-aux", aux, "-noQP", "-centers", f], tmp + "/out_" + name)
+        pack(name, tmp + "/out_" + name, {"input_positions": xy})
+
+
+def Golden_design(name):
+    """The reference's own smoke design (its .aux path)."""
+    return os.path.join(REF, "examples", "smoke", "smoke.aux")
+
+
 def qp_log(path):
     """The reference's per-iteration QP line (qplace.cpp:89): iteration,
     max(X error, Y error), HPWL."""
@@ -227,4 +277,4 @@ def flow_fixtures(tmp):
 
 
 if __name__ == "__main__":
-    main(set(sys.argv[1:]) or {"gp", "poisson", "legal", "qp", "mlg", "flow"})
+    main(set(sys.argv[1:]) or {"gp", "poisson", "legal", "qp", "mlg", "flow", "dp"})

@@ -47,6 +47,31 @@ def test_adapter_builds_against_reference_headers():
     assert r.returncode == 2 and "-aux is required" in r.stderr
 
 
+def test_detailed_placement_through_adapter(tmp_path):
+    """detailedPlacementAMD (ePlace_main.cpp:271-283) on a PlaceDB that the
+    reference's parser filled and its AbacusLegalizer legalised: the
+    library's result equals the reference DetailedPlacer's, run in the same
+    process from the same legal positions, bit for bit.  No device work
+    (-noGP), so this runs on the CPU."""
+    if not os.access(ADAPTER, os.X_OK):
+        pytest.skip("oracle/_ref/adapter_main is not built (it needs the reference tree at build time)")
+    g = Golden("dp_synth_c2")
+    d = g.design()
+    d.node_x[:] = g["input_positions"][:, 0]
+    d.node_y[:] = g["input_positions"][:, 1]
+    aux = write_bookshelf(d, str(tmp_path / "in"))
+    out = tmp_path / "out"
+    r = subprocess.run([ADAPTER, "-aux", aux, "-noQP", "-noGP", "-internalDP", "-out", str(out)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, (r.stdout[-2000:], r.stderr[-2000:])
+    s = dict((k, float(v)) for k, v in (ln.split() for ln in (out / "scalars.txt").read_text().splitlines()))
+    assert s["done"] == 1
+    assert s["dp_max_diff_vs_ref"] == 0.0
+    assert s["dp_hpwl"] == s["dp_ref_hpwl"]
+    assert s["dp_lib_hpwl_before"] == pytest.approx(s["legal_hpwl"], rel=1e-12)
+    assert s["dp_hpwl"] < s["legal_hpwl"]
+
+
 def run_adapter(tmp_path, name, args, timeout=300):
     if not os.access(ADAPTER, os.X_OK):
         pytest.skip("oracle/_ref/adapter_main is not built (it needs the reference tree at build time)")
