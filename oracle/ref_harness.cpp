@@ -9,8 +9,10 @@
 //
 // Modes
 //   gp   -aux F.aux -targetDensity TD [-targetOverflow T] [-bktrk] [-LSE]
-//        [-noQP] [-loadpl F.pl] [-iters K] [-dumpEvery M] [-mixed] [-legal] -out DIR
-//        (-mixed: for designs with macros, mLG, FILLERONLY and cGP after mGP)
+//        [-noQP] [-loadpl F.pl] [-iters K] [-dumpEvery M] [-mixed] [-legal]
+//        [-bins N] -out DIR
+//        (-mixed: for designs with macros, mLG, FILLERONLY and cGP after mGP;
+//        -bins: an N x N bin grid instead of the reference's bin rule)
 //   fft  -nx NX -ny NY -bx BX -by BY -in RHO.f32 -out DIR
 //   time -aux F.aux -targetDensity TD [-noQP] [-loadpl F.pl] [-iters K]
 //        (wall time of K vanilla iterations after initialisation, 1 thread)
@@ -247,6 +249,57 @@ void dump_eval(EPlacer_2D *ep, const std::string &tag) {
   scalar(tag + "last_hpwl", ep->lastHPWL);
 }
 
+// EPlacer_2D::binInitialization (eplace.cpp:221-412) for a given grid: the
+// reference derives the grid from the average node area and caps it at
+// 1024; the workload configurations name their grids (16x16 for 100 cells,
+// 1024x1024 and 2048x2048), so the harness builds the bins, the terminal
+// density and the base density for nx x ny bins the way that function
+// builds them for its own grid.
+void bin_initialization_at(EPlacer_2D *ep, PlaceDB *db, int nx, int ny) {
+  const float cw = db->coreRegion.getWidth(), ch = db->coreRegion.getHeight();
+  ep->binDimension.x = nx;
+  ep->binDimension.y = ny;
+  ep->binStep.x = float_div(cw, nx);
+  ep->binStep.y = float_div(ch, ny);
+  const float sx = ep->binStep.x, sy = ep->binStep.y;
+  ep->bins.assign(nx, std::vector<Bin_2D *>());
+  for (int i = 0; i < nx; i++) {
+    ep->bins[i].resize(ny);
+    for (int j = 0; j < ny; j++) {
+      Bin_2D *b = new Bin_2D();
+      b->ll.x = i * sx + db->coreRegion.ll.x;
+      b->ll.y = j * sy + db->coreRegion.ll.y;
+      b->width = sx;
+      b->height = sy;
+      b->ur.x = b->ll.x + b->width;
+      b->ur.y = b->ll.y + b->height;
+      b->area = sx * sy;
+      b->center.x = b->ll.x + (float)0.5 * b->width;
+      b->center.y = b->ll.y + (float)0.5 * b->height;
+      ep->bins[i][j] = b;
+    }
+  }
+  const float td = ep->targetDensity;
+  for (Module *t : db->dbTerminals) {
+    const POS_2D ll = t->getLL_2D(), ur = t->getUR_2D();
+    if (ll.x < db->coreRegion.ll.x || ll.x + t->getWidth() > db->coreRegion.ur.x) continue;
+    if (ll.y < db->coreRegion.ll.y || ll.y + t->getHeight() > db->coreRegion.ur.y) continue;
+    const int x0 = (int)((ll.x - db->coreRegion.ll.x) / sx), y0 = (int)((ll.y - db->coreRegion.ll.y) / sy);
+    const int x1 = std::min((int)((ur.x - db->coreRegion.ll.x) / sx), nx - 1);
+    const int y1 = std::min((int)((ur.y - db->coreRegion.ll.y) / sy), ny - 1);
+    for (int i = x0; i <= x1; i++)
+      for (int j = y0; j <= y1; j++)
+        ep->bins[i][j]->terminalDensity += td * getOverlapArea_2D(ep->bins[i][j]->ll, ep->bins[i][j]->ur, ll, ur);
+  }
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++) {
+      Bin_2D *b = ep->bins[i][j];
+      float avail = 0;
+      for (SiteRow r : db->dbSiteRows) avail += getOverlapArea_2D(b->ll, b->ur, r.getLL_2D(), r.getUR_2D());
+      b->baseDensity = float_equal(b->area, avail) ? 0 : td * (b->area - avail);
+    }
+}
+
 int mode_gp(int argc, char **argv) {
   BookshelfParser parser;
   PlaceDB *db = new PlaceDB();
@@ -260,7 +313,9 @@ int mode_gp(int argc, char **argv) {
   // lambda = 0, then lambda0 (eplace.cpp:39-50).  Run the parts one by one so
   // the first evaluation can be dumped before lambda0 is set.
   ep->fillerInitialization();
-  ep->binInitialization();
+  const int fixedBins = atoi(arg_str(argc, argv, "bins", "0").c_str());
+  if (fixedBins > 0) bin_initialization_at(ep, db, fixedBins, fixedBins);
+  else ep->binInitialization();
   ep->gradientVectorInitialization();
   npy_f32("fillers_init", positions(ep->ePlaceFillers), 2);
   std::vector<float> fw;

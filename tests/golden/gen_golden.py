@@ -1,7 +1,7 @@
 """Generate golden fixtures from the reference placer (run in the container
 that has the reference tree; the fixtures are committed, the reference is not).
 
-    python tests/golden/gen_golden.py [gp] [bb] [poisson] [legal] [qp] [mlg] [flow] [dp]   (default: all)
+    python tests/golden/gen_golden.py [gp] [bb] [poisson] [legal] [qp] [mlg] [flow] [dp] [grid]   (default: all)
 
 Builds oracle/_ref/ref_harness from the reference's own sources
 (oracle/Makefile.ref), runs it on the reference's smoke design and on seeded
@@ -43,9 +43,9 @@ def run(args, out):
                        cwd=tempfile.gettempdir())
 
 
-def pack(name, out, extra=None):
+def pack(name, out, extra=None, keep=None):
     arrays = {}
-    for k in KEEP:
+    for k in keep or KEEP:
         f = os.path.join(out, k + ".npy")
         if os.path.exists(f):
             arrays[k] = np.load(f)
@@ -74,6 +74,8 @@ def main(parts):
         legal_fixtures(tmp)
     if "dp" in parts:
         dp_fixtures(tmp)
+    if "grid" in parts:
+        grid_fixtures(tmp)
     if "qp" in parts:
         qp_fixtures(tmp)
     if "mlg" in parts:
@@ -106,6 +108,37 @@ def gp_fixtures(tmp):
         run(["gp", "-aux", aux, "-targetDensity", str(td), "-targetOverflow", "0.1", "-noQP",
              "-loadpl", aux[:-4] + ".pl", "-dumpEvery", "1000"], tmp + "/out_" + name)
         pack(name, tmp + "/out_" + name, {"target_density": td})
+
+
+def grid_fixtures(tmp):
+    # 8. the reference at the grids the workload configurations name, through
+    # the harness's copy of binInitialization for a given grid (-bins):
+    # config 1 as specified (100 cells, 16x16 bins, target density 0.8, QP ->
+    # mGP -> Abacus, then detailed placement from the GP result), and a
+    # scaled-down config 3 (20000 cells + 60 macros with sides of 3-12% of the
+    # die, utilisation 0.75, 256x256 bins: mGP -> mLG -> FILLERONLY -> cGP)
+    # seed 2: every cell has a pin (the reference's global swap needs that)
+    d = synthetic(num_cells=100, num_nets=184, seed=2, degrees="c1", widths="uniform14", name="c1_16")
+    aux = write_bookshelf(d, os.path.join(tmp, "c1_16"))
+    run(["gp", "-aux", aux, "-targetDensity", "0.8", "-targetOverflow", "0.1", "-bins", "16", "-legal",
+         "-dumpEvery", "1000"], tmp + "/out_c1_16")
+    gp_xy = np.load(tmp + "/out_c1_16/final_positions.npy")[:100]
+    f = os.path.join(tmp, "c1_16.f32")
+    np.ascontiguousarray(gp_xy, np.float32).tofile(f)
+    run(["dp", "-aux", aux, "-noQP", "-centers", f], tmp + "/dp_c1_16")
+    extra = {"target_density": 0.8, "bins": 16, "dp_positions": np.load(tmp + "/dp_c1_16/gs1.npy")}
+    for line in open(tmp + "/dp_c1_16/scalars.txt"):
+        k, v = line.split()
+        if k.startswith("dp_") or k.endswith("_hpwl"):
+            extra["dpflow_" + k] = float(v)
+    pack("c1_16_flow", tmp + "/out_c1_16", extra)
+    d = synthetic(num_cells=20000, num_nets=21000, seed=41, num_macros=60, util=0.75, name="mixed20k")
+    aux = write_bookshelf(d, os.path.join(tmp, "mixed20k"))
+    run(["gp", "-aux", aux, "-targetDensity", "1.0", "-targetOverflow", "0.1", "-noQP", "-loadpl",
+         aux[:-4] + ".pl", "-bins", "256", "-mixed", "-dumpEvery", "100000"], tmp + "/out_mixed20k")
+    pack("mixed20k_256", tmp + "/out_mixed20k", {"target_density": 1.0, "bins": 256},
+         keep=["nodes_whxy", "nodes_macro", "terms_whxyll", "terms_ni", "pin_obj", "pin_offset", "net_start", "rows",
+               "trajectory"])
 
 
 def bb_fixtures(tmp):
@@ -277,4 +310,4 @@ def flow_fixtures(tmp):
 
 
 if __name__ == "__main__":
-    main(set(sys.argv[1:]) or {"gp", "poisson", "legal", "qp", "mlg", "flow", "dp"})
+    main(set(sys.argv[1:]) or {"gp", "poisson", "legal", "qp", "mlg", "flow", "dp", "grid"})

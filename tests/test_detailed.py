@@ -91,3 +91,18 @@ def test_rejects_bad_arguments():
         detailed_place(d, g["legal_positions"][: d.num_nodes], passes=-1)
     with pytest.raises(RuntimeError):
         detailed_place(d, g["legal_positions"][: d.num_nodes], site_step=-1.0)
+
+
+def test_config1_grid_flow_tail_matches_reference():
+    """Config 1 as specified (100 cells, 16x16 bins, target density 0.8; the
+    reference run at that grid through the harness's fixed-grid bin set-up):
+    from the reference's GP result, Abacus and detailed placement give the
+    reference's legal and detailed positions bit for bit."""
+    g = Golden("c1_16_flow")
+    d = g.design()
+    legal, _ = legalize_abacus(d, g["final_positions"][: d.num_nodes], mode=0)
+    assert np.array_equal(legal, g["legal_positions"][: d.num_nodes])
+    out, st = detailed_place(d, legal)
+    assert np.array_equal(out, g["dp_positions"][: d.num_nodes])
+    assert st["hpwl_after"] == float(g["dpflow_dp_hpwl"])
+    assert check_legal(d, out)["legal"]
