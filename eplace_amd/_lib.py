@@ -109,6 +109,7 @@ class EPDPParams(C.Structure):
 
 class EPDPStats(C.Structure):
     _fields_ = [("hpwl_before", C.c_double), ("hpwl_after", C.c_double), ("seconds", C.c_double),
+                ("seconds_lr", C.c_double), ("seconds_ism", C.c_double), ("seconds_gs", C.c_double),
                 ("hpwl_after_lr", C.c_double * EP_DP_MAX_PASSES), ("hpwl_after_ism", C.c_double * EP_DP_MAX_PASSES),
                 ("hpwl_after_gs", C.c_double * EP_DP_MAX_PASSES), ("lr_windows", C.c_int64),
                 ("ism_matchings", C.c_int64), ("gs_moves", C.c_int64)]

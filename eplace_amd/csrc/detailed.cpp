@@ -115,6 +115,50 @@ struct Board {
     return s;
   }
 
+  // The boxes of node o's nets over the pins of other objects, one per pin
+  // of o in module order.  With them, module_hpwl at any position of o is
+  // moved_hpwl (min and max are exact, so adding o's own pins to these boxes
+  // gives the same floats as module_hpwl's walk over whole nets); the moves
+  // that try one node at many slots compute the boxes once.
+  struct Box {
+    float mnx, mxx, mny, mxy;
+  };
+  void other_boxes(int o, std::vector<Box> &out) const {
+    out.clear();
+    for (int k = npin_start[o]; k < npin_start[o + 1]; ++k) {
+      const int e = pin_net[npin[k]];
+      Box bx{kFloatMax, -kFloatMax, kFloatMax, -kFloatMax};
+      for (int64_t q = d->net_start[e]; q < d->net_start[e + 1]; ++q) {
+        if (d->pin_obj[q] == o) continue;
+        const float px = pin_x(q), py = pin_y(q);
+        bx.mnx = std::min(bx.mnx, px);
+        bx.mxx = std::max(bx.mxx, px);
+        bx.mny = std::min(bx.mny, py);
+        bx.mxy = std::max(bx.mxy, py);
+      }
+      out.push_back(bx);
+    }
+  }
+  double moved_hpwl(int o, const std::vector<Box> &boxes) const {
+    double s = 0;
+    const int k0 = npin_start[o], k1 = npin_start[o + 1];
+    for (int k = k0; k < k1; ++k) {
+      const int e = pin_net[npin[k]];
+      Box bx = boxes[k - k0];
+      for (int k2 = k0; k2 < k1; ++k2) {
+        const int q = npin[k2];
+        if (pin_net[q] != e) continue;
+        const float px = pin_x(q), py = pin_y(q);
+        bx.mnx = std::min(bx.mnx, px);
+        bx.mxx = std::max(bx.mxx, px);
+        bx.mny = std::min(bx.mny, py);
+        bx.mxy = std::max(bx.mxy, py);
+      }
+      s += ((bx.mxx - bx.mnx) + (bx.mxy - bx.mny)) + 0.0f;
+    }
+    return s;
+  }
+
   // PlaceDB::setModuleLocation_2D (placedb.cpp:203-236): clamped into the core.
   void set_ll(int o, float px, float py) {
     if (px < core_llx) px = core_llx;
@@ -239,15 +283,25 @@ std::vector<RowBook> row_books(const Board &b) {
 // paths after column reduction, reduction transfer and two rounds of
 // augmenting row reduction, on an integer cost matrix (lap2::lapSolve,
 // detailed.cpp:1117-1420, same steps and tie rules) ----
-std::vector<int> assign(const std::vector<std::vector<int>> &c) {
-  const int n = (int)c.size();
+struct Assign {
+  std::vector<int> rowsol, colsol, v, matches, freer, col, d, pred;
+};
+
+// c: n x n row-major; the result (the column of each row) is in w.rowsol.
+void assign(const int *cm, int n, Assign &w) {
+  auto c = [cm, n](int i, int j) { return cm[(size_t)i * n + j]; };
   const int inf = INT_MAX;
-  std::vector<int> rowsol(n, 0), colsol(n, 0), v(n, 0), matches(n, 0), freer(n, 0), col(n), d(n), pred(n);
+  for (std::vector<int> *a : {&w.rowsol, &w.colsol, &w.v, &w.matches, &w.freer}) a->assign(n, 0);
+  w.col.resize(n);
+  w.d.resize(n);
+  w.pred.resize(n);
+  std::vector<int> &rowsol = w.rowsol, &colsol = w.colsol, &v = w.v, &matches = w.matches, &freer = w.freer,
+                   &col = w.col, &d = w.d, &pred = w.pred;
   // column reduction, last column first
   for (int j = n - 1; j >= 0; --j) {
-    int mn = c[0][j], imin = 0;
+    int mn = c(0, j), imin = 0;
     for (int i = 1; i < n; ++i)
-      if (c[i][j] < mn) mn = c[i][j], imin = i;
+      if (c(i, j) < mn) mn = c(i, j), imin = i;
     v[j] = mn;
     if (++matches[imin] == 1) {
       rowsol[imin] = j;
@@ -265,7 +319,7 @@ std::vector<int> assign(const std::vector<std::vector<int>> &c) {
       const int j1 = rowsol[i];
       int mn = inf;
       for (int j = 0; j < n; ++j)
-        if (j != j1 && c[i][j] - v[j] < mn) mn = c[i][j] - v[j];
+        if (j != j1 && c(i, j) - v[j] < mn) mn = c(i, j) - v[j];
       v[j1] -= mn;
     }
   }
@@ -276,9 +330,9 @@ std::vector<int> assign(const std::vector<std::vector<int>> &c) {
     nfree = 0;
     while (k < prev) {
       const int i = freer[k++];
-      int umin = c[i][0] - v[0], j1 = 0, j2 = 0, usub = inf;
+      int umin = c(i, 0) - v[0], j1 = 0, j2 = 0, usub = inf;
       for (int j = 1; j < n; ++j) {
-        const int hh = c[i][j] - v[j];
+        const int hh = c(i, j) - v[j];
         if (hh < usub) {
           if (hh >= umin) {
             usub = hh;
@@ -310,7 +364,7 @@ std::vector<int> assign(const std::vector<std::vector<int>> &c) {
   for (int f = 0; f < nfree; ++f) {
     const int fr = freer[f];
     for (int j = 0; j < n; ++j) {
-      d[j] = c[fr][j] - v[j];
+      d[j] = c(fr, j) - v[j];
       pred[j] = fr;
       col[j] = j;
     }
@@ -341,10 +395,10 @@ std::vector<int> assign(const std::vector<std::vector<int>> &c) {
       if (!found) {
         const int j1 = col[low++];
         const int i = colsol[j1];
-        const int hh = c[i][j1] - v[j1] - mn;
+        const int hh = c(i, j1) - v[j1] - mn;
         for (int k = up; k < n; ++k) {
           const int j = col[k];
-          const int v2 = c[i][j] - v[j] - hh;
+          const int v2 = c(i, j) - v[j] - hh;
           if (v2 < d[j]) {
             pred[j] = i;
             if (v2 == mn) {
@@ -374,7 +428,6 @@ std::vector<int> assign(const std::vector<std::vector<int>> &c) {
       rowsol[i] = j1;
     } while (i != fr);
   }
-  return rowsol;
 }
 
 // ---- independent-set matching ----
@@ -384,6 +437,9 @@ struct Ism {
   int max_window = 90, max_cells = 128;  // ISMDP::initializeParams (detailed.cpp:761-774)
   bool double_window = false, independent = false;
   long long matchings = 0;
+  std::vector<int> cost;  // work space of the assignment, reused
+  Assign lap;
+  std::vector<Board::Box> boxes;
 
   explicit Ism(Board &bb) : b(bb), rows(row_books(bb)) {}
 
@@ -475,25 +531,27 @@ struct Ism {
       for (size_t i = 0; i < tail_x.size(); ++i) rows[tail_row[i]].insert_space(tail_x[i], tail_w[i]);
       for (int i = 0; i < taken; ++i) rows[taken_row[i]].remove_cell(b, who[i]);
       const int n = (int)who.size();
-      std::vector<std::vector<int>> cost(n, std::vector<int>(n, 0));
+      cost.assign((size_t)n * n, 0);
       for (int i = 0; i < n; ++i) {
         if (who[i] < 0) continue;
         const int o = who[i];
         float bx = sx[0], by = sy[0];
         double best = kDoubleMax;
+        b.other_boxes(o, boxes);
         for (int j = 0; j < n; ++j) {
           b.set_ll(o, sx[j], sy[j]);
-          const double wl = b.module_hpwl(o);
+          const double wl = b.moved_hpwl(o, boxes);
           if (wl < best) {
             best = wl;
             bx = sx[j];
             by = sy[j];
           }
-          cost[i][j] = (int)wl;
+          cost[(size_t)i * n + j] = (int)wl;
         }
         b.set_ll(o, bx, by);  // later cells are costed with this one at its best slot
       }
-      const std::vector<int> sol = assign(cost);
+      assign(cost.data(), n, lap);
+      const std::vector<int> &sol = lap.rowsol;
       ++matchings;
       for (int i = 0; i < n; ++i) {
         if (who[i] < 0) break;
@@ -696,6 +754,7 @@ struct Swap {
   std::vector<RowBook> rows;
   int max_slots = 30;  // GlobalSwapDP::initializeParams (detailed.cpp:2317-2325)
   long long moved = 0;
+  std::vector<Board::Box> boxes;
 
   explicit Swap(Board &bb) : b(bb), rows(row_books(bb)) {}
 
@@ -788,9 +847,10 @@ struct Swap {
       rows[orow].remove_cell(b, o);
       const double old = b.module_hpwl(o);
       bool done = false;
+      if (!ex.empty()) b.other_boxes(o, boxes);
       for (size_t s = 0; s < ex.size(); ++s) {
         b.set_ll(o, ex[s], ey[s]);
-        if (old > b.module_hpwl(o)) {
+        if (old > b.moved_hpwl(o, boxes)) {
           done = true;
           rows[b.row_of(ey[s])].insert_cell(b, ex[s], o);
           break;
@@ -885,17 +945,26 @@ int ep_detailed_place(const ep_design *d, const float *node_xy, const ep_dp_para
   ep_dp_stats st = {};
   st.hpwl_before = b.hpwl();
   const double limit = p->ism_time_limit > 0 ? p->ism_time_limit : 28800.0;
+  auto since = [](std::chrono::steady_clock::time_point t) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
+  };
   for (int pass = 0; pass < p->passes; ++pass) {
+    auto t = std::chrono::steady_clock::now();
     Reorder lr(b);
     for (int k = 0; k < 2; ++k) lr.solve(3, 2);
     st.lr_windows += lr.windows;
     if (pass < EP_DP_MAX_PASSES) st.hpwl_after_lr[pass] = b.hpwl();
+    st.seconds_lr += since(t);
+    t = std::chrono::steady_clock::now();
     st.ism_matchings += run_ism(b, limit);
     if (pass < EP_DP_MAX_PASSES) st.hpwl_after_ism[pass] = b.hpwl();
+    st.seconds_ism += since(t);
+    t = std::chrono::steady_clock::now();
     Swap gs(b);
     for (int k = 0; k < 2; ++k) gs.solve();
     st.gs_moves += gs.moved;
     if (pass < EP_DP_MAX_PASSES) st.hpwl_after_gs[pass] = b.hpwl();
+    st.seconds_gs += since(t);
   }
   st.hpwl_after = b.hpwl();
   st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -69,8 +69,8 @@ def detailed_place(d: Design, xy=None, passes: int = 2, site_step: float | None 
         raise L.EPError(rc, lib.ep_host_last_error().decode())
     del keep
     n = min(int(passes), L.EP_DP_MAX_PASSES)
-    stats = {k: getattr(st, k) for k in ("hpwl_before", "hpwl_after", "seconds", "lr_windows", "ism_matchings",
-                                         "gs_moves")}
+    stats = {k: getattr(st, k) for k in ("hpwl_before", "hpwl_after", "seconds", "seconds_lr", "seconds_ism",
+                                         "seconds_gs", "lr_windows", "ism_matchings", "gs_moves")}
     for k in ("hpwl_after_lr", "hpwl_after_ism", "hpwl_after_gs"):
         stats[k] = list(getattr(st, k))[:n]
     return out, stats

@@ -395,6 +395,7 @@ typedef struct ep_dp_params {
 
 typedef struct ep_dp_stats {
   double hpwl_before, hpwl_after, seconds;
+  double seconds_lr, seconds_ism, seconds_gs; /* time in each move over all rounds */
   double hpwl_after_lr[EP_DP_MAX_PASSES], hpwl_after_ism[EP_DP_MAX_PASSES], hpwl_after_gs[EP_DP_MAX_PASSES];
   int64_t lr_windows, ism_matchings, gs_moves;
 } ep_dp_stats;
