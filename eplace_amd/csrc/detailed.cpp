@@ -939,6 +939,16 @@ int ep_detailed_place(const ep_design *d, const float *node_xy, const ep_dp_para
     for (int64_t q = 0; q < P; ++q)
       if (d->pin_obj[q] < b.N) b.npin[cur[d->pin_obj[q]]++] = (int)q;
   }
+  // the moves index rows by each cell's lower edge: the input must be a
+  // legal placement (every standard cell inside the core, on a row)
+  for (int o = 0; o < b.N; ++o) {
+    if (b.macro[o]) continue;
+    const int r = b.row_of(b.y[o]);
+    if (!(b.x[o] >= b.core_llx && b.x[o] + b.w[o] <= b.core_urx) || r < 0 || r >= b.R ||
+        (double)b.y[o] != b.row_bottom[r])
+      return err = "node " + std::to_string(o) + " is not on a row inside the core: legalise before detailed placement",
+             EP_ERR_ARG;
+  }
   // DetailedPlacer::initialization: the interval removal once more after the
   // legaliser's (placedb.cpp:629-849 applied twice)
   if (!row_intervals(d, g, p->site_step, b.iv, err, 2)) return EP_ERR_ARG;

@@ -84,6 +84,14 @@ def test_cells_without_nets_stay_legal():
     assert st["hpwl_after"] < st["hpwl_before"]
 
 
+def test_rejects_a_placement_that_is_not_legal():
+    """Cells off the rows (a global placement) are refused before any move."""
+    g = Golden("dp_synth_c2")
+    d = g.design()
+    with pytest.raises(RuntimeError, match="legalise before detailed placement"):
+        detailed_place(d, g["input_positions"][: d.num_nodes])
+
+
 def test_rejects_bad_arguments():
     g = Golden("dp_smoke")
     d = g.design()
