@@ -86,7 +86,7 @@ class EPRunStats(C.Structure):
         ("iterations", C.c_int32), ("evaluations", C.c_int32), ("backtracks", C.c_int32),
         ("stopped", C.c_int32), ("final_overflow", C.c_double), ("final_hpwl", C.c_double),
         ("seconds", C.c_double), ("controller_fallbacks", C.c_int32), ("bb_short_steps", C.c_int32),
-        ("gemm_seconds", C.c_double), ("gemm_launches", C.c_int32), ("pad", C.c_int32),
+        ("gemm_seconds", C.c_double), ("gemm_launches", C.c_int32), ("numeric_fallbacks", C.c_int32),
     ]
 
 

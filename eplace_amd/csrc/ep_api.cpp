@@ -928,6 +928,7 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
     out->bb_short_steps = h.bb_short;
     out->gemm_seconds = (double)h.gemm_ticks / (1e3 * c.wall_khz);
     out->gemm_launches = h.gemm_launches;
+    out->numeric_fallbacks = h.bad_numeric;
   }
   return EP_OK;
 }

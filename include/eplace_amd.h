@@ -119,7 +119,8 @@ typedef struct ep_opt_params {
                                else min(|s|/|y|, 1/L) (nesterov.hpp:231-302) */
   float target_overflow;    /* mGP stop threshold (default 0.10) */
   int32_t max_iterations;   /* MAX_ITERATION (1000); stop when iter > this */
-  int32_t max_backtracks;   /* cap on step retries per iteration (reference: none) */
+  int32_t max_backtracks;   /* cap on trial steps per iteration, the last taken when every
+                               one fails the acceptance test (30; reference: no cap) */
   int32_t sync_every;       /* host polls the stop flag every this many iterations */
   int32_t log_capacity;     /* rows available in log (may be 0) */
   double *log;              /* optional [log_capacity][EP_LOG_COLS], columns EP_LOG_* */
@@ -149,7 +150,10 @@ typedef struct ep_run_stats {
                                wave's end on the device's wall clock (the last evaluation's
                                launches are not included) */
   int32_t gemm_launches;    /* launches summed in gemm_seconds */
-  int32_t pad;
+  int32_t numeric_fallbacks; /* steps or acceptance tests whose 1/L was not finite and
+                                positive (identical consecutive positions): the previous
+                                step was kept, or the trial accepted (the reference would
+                                carry the NaN/inf into the positions) */
 } ep_run_stats;
 
 /* Which device array ep_read/ep_write address (parity and debugging). */
