@@ -4,6 +4,7 @@
 //   ep_place -aux <design.aux> [-targetDensity 1.0] [-targetOverflow 0.10]
 //            [-loadpl <file.pl>] [-noQP] [-IPiteCount <n>] [-bktrk] [-bb] [-LSE]
 //            [-bins <n>] [-nomGP] [-internalLegal] [-internalDP] [-outputPath <dir>]
+//            [-logCSV <file>]
 //            [-dumpDesign <file>]
 //
 // Reads the Bookshelf design, runs QP (QPPlacer::quadraticPlacement from the
@@ -19,6 +20,9 @@
 // -internalDP then runs the detailed placer on the legal result
 // (DetailedPlacer::detailedPlacement, ePlace_main.cpp:271-283), checks it
 // and writes <name>-eDP.pl.
+// -logCSV writes one row per Nesterov iteration of every stage (stage,
+// iteration, overflow, lambda, HPWL, step, 1/gamma_x, evaluations and
+// backtracks so far), the fields the reference prints per iteration.
 // -dumpDesign writes the parsed arrays (ep_design field order, raw
 // little-endian) to a file and exits before any device work.
 #include <sys/stat.h>
@@ -37,7 +41,7 @@ using namespace eplace_amd;
 namespace {
 
 struct Args {
-  std::string aux, loadpl, out = ".", dump;
+  std::string aux, loadpl, out = ".", dump, log_csv;
   float td = 1.0f, tof = 0.10f;
   bool bktrk = false, bb = false, lse = false, no_mgp = false, legal = false, dp = false, no_qp = false, no_mlg = false, no_cgp = false;
   int bins = 0, qp_iters = 20;
@@ -49,7 +53,7 @@ int usage(const char *prog) {
   fprintf(stderr,
           "usage: %s -aux <design.aux> [-targetDensity d] [-targetOverflow t] [-loadpl f.pl]\n"
           "          [-noQP] [-IPiteCount n] [-bktrk] [-bb] [-LSE] [-bins n] [-nomGP] [-nomLG] [-nocGP]\n"
-          "          [-internalLegal] [-internalDP]\n"
+          "          [-internalLegal] [-internalDP] [-logCSV file]\n"
           "          [-outputPath dir] [-dumpDesign file]\n",
           prog);
   return 2;
@@ -93,6 +97,8 @@ bool parse(int argc, char **argv, Args &a) {
       a.legal = true;
     } else if (k == "-internalDP") {
       a.dp = true;
+    } else if (k == "-logCSV") {
+      if (!val(a.log_csv)) return false;
     } else if (k == "-IPiteCount") {
       if (!val(v)) return false;
       a.qp_iters = atoi(v.c_str());
@@ -178,9 +184,26 @@ int main(int argc, char **argv) {
     ep_info info = placer.info();
     printf("bins %dx%d, %lld fillers, target density %.3f\n", info.bins_x, info.bins_y, (long long)info.num_fillers, a.td);
 
+    FILE *csv = nullptr;
+    if (!a.log_csv.empty()) {
+      csv = fopen(a.log_csv.c_str(), "w");
+      if (!csv) throw std::runtime_error("cannot write " + a.log_csv);
+      fprintf(csv, "stage,iteration,overflow,lambda,hpwl,step,inv_gamma_x,evaluations,backtracks\n");
+    }
+    // one CSV row per iteration of a stage's run (EP_LOG_* columns)
+    auto log_rows = [&](const char *stage, const EplaceNesterovOpt<VECTOR_3D> &o) {
+      if (!csv) return;
+      const std::vector<double> &g = o.log();
+      for (size_t r = 0; r + EP_LOG_COLS <= g.size(); r += EP_LOG_COLS)
+        fprintf(csv, "%s,%d,%.9g,%.9g,%.17g,%.9g,%.9g,%d,%d\n", stage, (int)g[r + EP_LOG_ITER], g[r + EP_LOG_TAU],
+                g[r + EP_LOG_LAMBDA], g[r + EP_LOG_HPWL], g[r + EP_LOG_STEP], g[r + EP_LOG_INV_GAMMA],
+                (int)g[r + EP_LOG_EVALS], (int)g[r + EP_LOG_BACKTRACKS]);
+    };
     if (!a.no_mgp) {
       EplaceNesterovOpt<VECTOR_3D> opt(&placer, a.method(), a.tof);
+      opt.keepLog(csv != nullptr);
       opt.opt();
+      log_rows("mGP", opt);
       const ep_run_stats &s = opt.last();
       printf("mGP finished: %d iterations, %d evaluations, overflow %.4f\n", s.iterations, s.evaluations,
              s.final_overflow);
@@ -200,11 +223,15 @@ int main(int argc, char **argv) {
       if (!a.no_cgp) {
         placer.switch2FillerOnly();
         EplaceNesterovOpt<VECTOR_3D> fo(&placer, a.method(), a.tof);
+        fo.keepLog(csv != nullptr);
         fo.opt();
+        log_rows("FILLERONLY", fo);
         printf("filler placement finished: %d iterations\n", fo.last().iterations);
         placer.switch2cGP();
         EplaceNesterovOpt<VECTOR_3D> cg(&placer, a.method(), a.tof);
+        cg.keepLog(csv != nullptr);
         cg.opt();
+        log_rows("cGP", cg);
         const ep_run_stats &s = cg.last();
         printf("cGP finished: %d iterations, overflow %.4f\n", s.iterations, s.final_overflow);
         printf("cGP Final HPWL: %.6e\n", s.final_hpwl);
@@ -217,6 +244,10 @@ int main(int argc, char **argv) {
     std::vector<float> all(2 * (size_t)info.num_objects);
     placer.check(ep_read(placer.context(), EP_ARR_POSITIONS, all.data(), (int64_t)all.size()));
     std::copy(all.begin(), all.begin() + xy.size(), xy.begin());
+    if (csv) {
+      fclose(csv);
+      printf("wrote %s\n", a.log_csv.c_str());
+    }
     std::string dir = a.out + "/" + d.name + "/";
     mkdirs(dir);
     std::string pl = dir + d.name + "-eGP.pl";

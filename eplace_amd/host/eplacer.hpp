@@ -9,6 +9,7 @@
 // std::runtime_error carrying ep_last_error(); nothing calls exit().
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -212,16 +213,30 @@ class EplaceNesterovOpt : public FirstOrderOptimizer<T> {
     p.log_capacity = 0;
     p.log = nullptr;
     p.phase_seconds = nullptr;
+    if (keepLog_) {
+      log_.assign((size_t)(p.max_iterations + 2) * EP_LOG_COLS, 0.0);
+      p.log_capacity = p.max_iterations + 2;
+      p.log = log_.data();
+    }
     placer_->check(ep_nesterov_run(placer_->context(), &p, &last_));
+    if (keepLog_) log_.resize((size_t)std::min(last_.iterations, p.log_capacity) * EP_LOG_COLS);
     if (placer_->placementStage == mGP) placer_->mGPIterationCount = last_.iterations;
   }
   const ep_run_stats &last() const { return last_; }
+  // The per-iteration log of the next opt() (EP_LOG_COLS doubles per
+  // iteration, columns EP_LOG_*): what the reference prints per iteration
+  // (nesterov.hpp:154, 333, 395) plus evaluations, backtracks and the
+  // controller's inputs.
+  void keepLog(bool on) { keepLog_ = on; }
+  const std::vector<double> &log() const { return log_; }
 
  private:
   EPlacer_2D *placer_;
   int method_;
   float targetOverflow_;
   ep_run_stats last_{};
+  bool keepLog_ = false;
+  std::vector<double> log_;
 };
 
 }  // namespace eplace_amd

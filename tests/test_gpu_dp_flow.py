@@ -73,8 +73,9 @@ def test_cli_internal_dp(tmp_path):
     d = synthetic(num_cells=2000, num_nets=2100, seed=12, name="dpcli")
     aux = write_bookshelf(d, str(tmp_path / "in"))
     say("[dp flow] ep_place -internalLegal -internalDP on 2000 cells")
+    csv = tmp_path / "log.csv"
     r = subprocess.run([EXE, "-aux", aux, "-bins", "32", "-internalLegal", "-internalDP", "-outputPath",
-                        str(tmp_path / "out")], capture_output=True, text=True, timeout=120)
+                        str(tmp_path / "out"), "-logCSV", str(csv)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     lines = r.stdout.splitlines()
     before = float([ln for ln in lines if ln.startswith("HPWL before detailed placement")][0].split()[-1])
@@ -86,3 +87,12 @@ def test_cli_internal_dp(tmp_path):
     e = read_bookshelf(aux)
     load_pl(e, str(pl))
     assert check_legal(e)["legal"]
+    # the per-iteration log: one mGP row per iteration, the overflow falling
+    # to the target at the last row
+    rows = csv.read_text().splitlines()
+    assert rows[0].startswith("stage,iteration,overflow,lambda,hpwl")
+    mgp = [ln.split(",") for ln in rows[1:] if ln.startswith("mGP,")]
+    iters = int([ln for ln in lines if ln.startswith("mGP finished")][0].split()[2])
+    assert len(mgp) == iters
+    assert [int(m[1]) for m in mgp] == list(range(iters))
+    assert float(mgp[-1][2]) <= 0.10 < float(mgp[0][2])
