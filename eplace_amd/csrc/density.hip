@@ -629,46 +629,6 @@ void launch_finalize_density(Ctx &c) {
                      c.partials, c.sc);
 }
 
-// Node wirelength gradients: each node's pin gradients, read through
-// node_pin_lane in modulePins order (objects.cpp:380-415 summed per module,
-// eplace.cpp:694-737).  Runs on the wiring stream, off the critical path.
-__global__ void __launch_bounds__(kBlock) node_wl_kernel(int N, const int32_t *__restrict__ node_pin_start,
-                                                          const int32_t *__restrict__ node_pin_lane,
-                                                          const float2 *__restrict__ pin_grad, float2 *gwl,
-                                                          const DevScalars *sc) {
-  if (sc->done) return;
-  const int o = xcd_block() * blockDim.x + threadIdx.x;
-  if (o >= N) return;
-  float2 wl = make_float2(0.f, 0.f);
-  int q = node_pin_start[o];
-  const int e = node_pin_start[o + 1];
-  // four gathers in flight, summed in pin order
-  for (; q + 4 <= e; q += 4) {
-    const int l0 = node_pin_lane[q], l1 = node_pin_lane[q + 1], l2 = node_pin_lane[q + 2],
-              l3 = node_pin_lane[q + 3];
-    const float2 p0 = pin_grad[l0], p1 = pin_grad[l1], p2 = pin_grad[l2], p3 = pin_grad[l3];
-    wl.x += p0.x;
-    wl.y += p0.y;
-    wl.x += p1.x;
-    wl.y += p1.y;
-    wl.x += p2.x;
-    wl.y += p2.y;
-    wl.x += p3.x;
-    wl.y += p3.y;
-  }
-  for (; q < e; ++q) {
-    const float2 pg = pin_grad[node_pin_lane[q]];
-    wl.x += pg.x;
-    wl.y += pg.y;
-  }
-  gwl[o] = wl;
-}
-
-void launch_node_wl(Ctx &c, hipStream_t st) {
-  hipLaunchKernelGGL(node_wl_kernel, dim3(object_blocks(c.N)), dim3(kBlock), 0, st, c.N, c.node_pin_start,
-                     c.node_pin_lane, c.pin_grad, c.gwl, c.sc);
-}
-
 void launch_field_gather(Ctx &c) {
   Grid g = c.grid();
   int ldp = poisson_ld(c);

@@ -547,8 +547,7 @@ static void enqueue_evaluation(Ctx &c, bool lambda_update, bool lipschitz, const
   if (ev) (void)hipEventRecord(ev[6], c.stream2);
   launch_wirelength(c, true, c.stream2);
   if (ev) (void)hipEventRecord(ev[8], c.stream2);
-  launch_reduce_wirelength(c, next_lambda ? 2 + next_lambda : lambda_update ? 1 : 0, c.stream2);
-  launch_node_wl(c, c.stream2);
+  launch_node_wl_reduce(c, next_lambda ? 2 + next_lambda : lambda_update ? 1 : 0, c.stream2);
   if (ev) (void)hipEventRecord(ev[7], c.stream2);
   (void)hipEventRecord(c.ev_join, c.stream2);
   launch_field_gather(c);

@@ -313,7 +313,6 @@ void launch_gather_combine(Ctx &c, bool recompute_gather, bool lipschitz, const 
                            const float2 *ref_prev, const float2 *grad_prev, float2 *pos_out = nullptr,
                            float2 *gtot_out = nullptr);
 void launch_field_gather(Ctx &c);
-void launch_node_wl(Ctx &c, hipStream_t st);
 void launch_combine(Ctx &c, bool recompute, bool lipschitz, const float2 *ref_now, const float2 *ref_prev,
                     const float2 *grad_prev, float2 *pos_out = nullptr, float2 *gtot_out = nullptr,
                     bool at_next_lambda = false);
@@ -327,6 +326,9 @@ void launch_wirelength(Ctx &c, bool gradient, hipStream_t st = nullptr);
 // controller's floor; 3 and 4 stage 1 and 2 in lambda_next / ref_mult_next
 // without applying them.
 void launch_reduce_wirelength(Ctx &c, int lambda_update, hipStream_t st = nullptr, bool with_value = true);
+// the node sums of the pin gradients with the same reduction (with_value)
+// in block 0 of the launch
+void launch_node_wl_reduce(Ctx &c, int lambda_update, hipStream_t st);
 // ---- poisson.hip ----
 bool poisson_create(Ctx &c, std::string &err);
 void poisson_destroy(Ctx &c);

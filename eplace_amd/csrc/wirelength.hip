@@ -377,8 +377,8 @@ __device__ void fold_gemm_stamps(const GemmStamps &st, DevScalars *sc) {
   }
 }
 
-__global__ void __launch_bounds__(kFinish) reduce_wl_kernel(int nslots, double *partials, DevScalars *sc,
-                                                            int lambda_update, int with_value, GemmStamps st) {
+__device__ void reduce_wl_block(int nslots, double *partials, DevScalars *sc, int lambda_update, int with_value,
+                                const GemmStamps &st) {
   if (st.t) fold_gemm_stamps(st, sc);
   if (sc->done) return;
   const double *const src[2] = {partials + kPartialWL, partials + kPartialWL + 1};
@@ -406,6 +406,57 @@ __global__ void __launch_bounds__(kFinish) reduce_wl_kernel(int nslots, double *
       }
     }
   }
+}
+
+__global__ void __launch_bounds__(kFinish) reduce_wl_kernel(int nslots, double *partials, DevScalars *sc,
+                                                            int lambda_update, int with_value, GemmStamps st) {
+  reduce_wl_block(nslots, partials, sc, lambda_update, with_value, st);
+}
+
+// Node wirelength gradients: each node's pin gradients, read through
+// node_pin_lane in modulePins order (objects.cpp:380-415 summed per module,
+// eplace.cpp:694-737).  Block 0 is the WL reduction above (the same block
+// size, so the same summation order): it needs the wiring kernels' partials
+// as the node sums need their pin gradients, and as a block of this launch
+// it runs beside the node sums instead of before them.  The other blocks
+// take 1024 objects each in XCD-contiguous order.
+__global__ void __launch_bounds__(kFinish) node_wl_kernel(int N, const int32_t *__restrict__ node_pin_start,
+                                                           const int32_t *__restrict__ node_pin_lane,
+                                                           const float2 *__restrict__ pin_grad, float2 *gwl,
+                                                           DevScalars *sc, int nslots, double *partials,
+                                                           int lambda_update, GemmStamps st) {
+  if (blockIdx.x == 0) {
+    reduce_wl_block(nslots, partials, sc, lambda_update, 1, st);
+    return;
+  }
+  if (sc->done) return;
+  const int nb = gridDim.x - 1, b = blockIdx.x - 1;
+  const int blk = nb % kXcds ? b : (b % kXcds) * (nb / kXcds) + b / kXcds;
+  const int o = blk * blockDim.x + threadIdx.x;
+  if (o >= N) return;
+  float2 wl = make_float2(0.f, 0.f);
+  int q = node_pin_start[o];
+  const int e = node_pin_start[o + 1];
+  // four gathers in flight, summed in pin order
+  for (; q + 4 <= e; q += 4) {
+    const int l0 = node_pin_lane[q], l1 = node_pin_lane[q + 1], l2 = node_pin_lane[q + 2],
+              l3 = node_pin_lane[q + 3];
+    const float2 p0 = pin_grad[l0], p1 = pin_grad[l1], p2 = pin_grad[l2], p3 = pin_grad[l3];
+    wl.x += p0.x;
+    wl.y += p0.y;
+    wl.x += p1.x;
+    wl.y += p1.y;
+    wl.x += p2.x;
+    wl.y += p2.y;
+    wl.x += p3.x;
+    wl.y += p3.y;
+  }
+  for (; q < e; ++q) {
+    const float2 pg = pin_grad[node_pin_lane[q]];
+    wl.x += pg.x;
+    wl.y += pg.y;
+  }
+  gwl[o] = wl;
 }
 
 static int g_small_blocks(const Ctx &c) {
@@ -460,6 +511,14 @@ void launch_reduce_wirelength(Ctx &c, int lambda_update, hipStream_t st, bool wi
   int n = g_small_blocks(c) + (c.num_large ? g_large_blocks(c) : 0);
   hipLaunchKernelGGL(reduce_wl_kernel, dim3(1), dim3(kFinish), 0, st, n, c.partials, c.sc, lambda_update,
                      with_value ? 1 : 0, c.gemm_stamps);
+}
+
+void launch_node_wl_reduce(Ctx &c, int lambda_update, hipStream_t st) {
+  const int n = g_small_blocks(c) + (c.num_large ? g_large_blocks(c) : 0);
+  int b = (c.N + kFinish - 1) / kFinish;
+  b = (b + kXcds - 1) / kXcds * kXcds;
+  hipLaunchKernelGGL(node_wl_kernel, dim3(b + 1), dim3(kFinish), 0, st, c.N, c.node_pin_start, c.node_pin_lane,
+                     c.pin_grad, c.gwl, c.sc, n, c.partials, lambda_update, c.gemm_stamps);
 }
 
 }  // namespace ep
