@@ -89,7 +89,7 @@ PHASE_KERNELS = {
     "scatter": "tile_hist_kernel + tile_place_kernel + tile_scatter_kernel + scatter_big_kernel",
     "finalize": "finalize_density_kernel + finish_density_kernel",
     "wirelength": "wl_group_kernel<0,0,1> + wl_large_kernel<0,0,1>",
-    "node_sums": "reduce_wl_kernel + node_wl_kernel",
+    "node_sums": "node_wl_kernel (block 0 runs the WL reduction)",
     "field_gather": "gather_tile_kernel + gather_big_kernel",
     "combine": "gather_combine_kernel (after the second stream joins)",
 }
