@@ -408,7 +408,7 @@ __device__ void reduce_wl_block(int nslots, double *partials, DevScalars *sc, in
   }
 }
 
-__global__ void __launch_bounds__(kFinish) reduce_wl_kernel(int nslots, double *partials, DevScalars *sc,
+__global__ void __launch_bounds__(kBlock) reduce_wl_kernel(int nslots, double *partials, DevScalars *sc,
                                                             int lambda_update, int with_value, GemmStamps st) {
   reduce_wl_block(nslots, partials, sc, lambda_update, with_value, st);
 }
@@ -419,8 +419,10 @@ __global__ void __launch_bounds__(kFinish) reduce_wl_kernel(int nslots, double *
 // size, so the same summation order): it needs the wiring kernels' partials
 // as the node sums need their pin gradients, and as a block of this launch
 // it runs beside the node sums instead of before them.  The other blocks
-// take 1024 objects each in XCD-contiguous order.
-__global__ void __launch_bounds__(kFinish) node_wl_kernel(int N, const int32_t *__restrict__ node_pin_start,
+// take 256 objects each in XCD-contiguous order.  Blocks of 256 threads
+// find room beside the transform GEMM and the field gather on the other
+// stream sooner than blocks of 1024.
+__global__ void __launch_bounds__(kBlock) node_wl_kernel(int N, const int32_t *__restrict__ node_pin_start,
                                                            const int32_t *__restrict__ node_pin_lane,
                                                            const float2 *__restrict__ pin_grad, float2 *gwl,
                                                            DevScalars *sc, int nslots, double *partials,
@@ -509,15 +511,15 @@ void launch_wirelength(Ctx &c, bool gradient, hipStream_t st) {
 void launch_reduce_wirelength(Ctx &c, int lambda_update, hipStream_t st, bool with_value) {
   if (!st) st = c.stream;
   int n = g_small_blocks(c) + (c.num_large ? g_large_blocks(c) : 0);
-  hipLaunchKernelGGL(reduce_wl_kernel, dim3(1), dim3(kFinish), 0, st, n, c.partials, c.sc, lambda_update,
+  hipLaunchKernelGGL(reduce_wl_kernel, dim3(1), dim3(kBlock), 0, st, n, c.partials, c.sc, lambda_update,
                      with_value ? 1 : 0, c.gemm_stamps);
 }
 
 void launch_node_wl_reduce(Ctx &c, int lambda_update, hipStream_t st) {
   const int n = g_small_blocks(c) + (c.num_large ? g_large_blocks(c) : 0);
-  int b = (c.N + kFinish - 1) / kFinish;
+  int b = (c.N + kBlock - 1) / kBlock;
   b = (b + kXcds - 1) / kXcds * kXcds;
-  hipLaunchKernelGGL(node_wl_kernel, dim3(b + 1), dim3(kFinish), 0, st, c.N, c.node_pin_start, c.node_pin_lane,
+  hipLaunchKernelGGL(node_wl_kernel, dim3(b + 1), dim3(kBlock), 0, st, c.N, c.node_pin_start, c.node_pin_lane,
                      c.pin_grad, c.gwl, c.sc, n, c.partials, lambda_update, c.gemm_stamps);
 }
 
