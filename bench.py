@@ -340,7 +340,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine runs global placement on one GPU; multi-GPU runs are not built "
-              "(see DESIGN.md for how the path would shard).", file=sys.stderr)
+              "(the sharding of the path is a design only).", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_probe:

@@ -1,6 +1,6 @@
 """Failure handling of the device-resident loop, forced from the host: every
 kernel still runs to its normal end.  The reference has none of these
-guards (SURVEY's failure-detection row): it carries a 0/0 Lipschitz
+guards: it carries a 0/0 Lipschitz
 constant into the positions (optutils.hpp:33-34), has no cap on
 backtracking (nesterov.hpp:337-364) and only its MAX_ITERATION cap
 (nesterov.hpp:36, 105)."""
