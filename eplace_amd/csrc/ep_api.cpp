@@ -833,9 +833,9 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
       } else if (k == 0) {
         launch_gather_combine(c, false, true, nullptr, c.last_ref, c.last_grad, c.cur_ref, c.cur_grad);
         launch_step_scalar(c, 0);
-      } else {
-        launch_step_scalar(c, 0, kPartialLipNext);
       }
+      // from the second iteration on, the step came with the stop check
+      // at the end of the previous iteration (launch_done_step)
       if (features) launch_ctrl_step(c, ctrl, learned ? 1 : 0);
       int tries = 0;
       bool stopped = false;
@@ -868,7 +868,8 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
       std::swap(c.cur_main, c.new_main);
       // updatePenaltyFactor with the HPWL of the last trial (or the
       // controller's floor), as staged by that trial, then the stop check
-      launch_done_check_ex(c, k + 1, dlog, cap, 1, 1);
+      // and the next iteration's step from that trial's Lipschitz sums
+      launch_done_step(c, k + 1, dlog, cap, kPartialLipNext);
       ++k;
       continue;
     }

@@ -345,6 +345,9 @@ void launch_copy_float2(Ctx &c, float2 *dst, const float2 *src, int64_t n);
 void launch_publish_scalars(Ctx &c, unsigned int seq);
 void launch_hold(Ctx &c, unsigned long long ticks);
 void launch_done_check_ex(Ctx &c, int next_iter, double *log, int log_cap, int set_iter, int apply_next = 0);
+// launch_done_check_ex(next_iter, log, log_cap, 1, 1) and then launch_step_scalar(0, lip_slot)
+// in one launch
+void launch_done_step(Ctx &c, int next_iter, double *log, int log_cap, int lip_slot);
 void launch_nesterov_move_ex(Ctx &c, const float2 *ref, const float2 *grad, const float2 *main_in,
                              float2 *main_out, float2 *ref_unclamped_out, float2 *save_ref, float2 *save_grad);
 int poisson_ld(const Ctx &c);

@@ -46,8 +46,7 @@ __global__ void copy_float2_kernel(int64_t n, float2 *dst, const float2 *src) {
 // step of opt_step_bb (nesterov.hpp:245-281): the short step s.y / y.y when
 // positive, else min(|s| / |y|, 1/L), s = y_k - y_{k-1}, y = g_k - g_{k-1}.
 // Also the Nesterov coefficient a_{k+1} and the controller's gradient norms.
-__global__ void __launch_bounds__(kFinish) step_scalar_kernel(int nblocks, double *partials, DevScalars *sc,
-                                                              int bb, int lip_slot) {
+__device__ void step_scalar_block(int nblocks, double *partials, DevScalars *sc, int bb, int lip_slot) {
   if (sc->done) return;
   const double *const src[5] = {partials + lip_slot, partials + lip_slot + 1, partials + kPartialNorm,
                                 partials + kPartialNorm + 1, partials + kPartialSY};
@@ -83,6 +82,11 @@ __global__ void __launch_bounds__(kFinish) step_scalar_kernel(int nblocks, doubl
     sc->a_next = an;  // committed by done_check
     sc->coef = (a - 1.0) / an;
   }
+}
+
+__global__ void __launch_bounds__(kFinish) step_scalar_kernel(int nblocks, double *partials, DevScalars *sc,
+                                                              int bb, int lip_slot) {
+  step_scalar_block(nblocks, partials, sc, bb, lip_slot);
 }
 
 // Backtracking acceptance (nesterov.hpp:337-364): accept when
@@ -138,8 +142,8 @@ __global__ void __launch_bounds__(kBlock) nesterov_move_kernel(
 // End of an iteration: commit a_{k+1}, advance the counter, evaluate the
 // stop condition for the next iteration (nesterov.hpp:86-149) with the tau
 // of the last evaluation, append a log row.
-__global__ void done_check_kernel(DevScalars *sc, int next_iter, double *log, int log_cap, int set_iter,
-                                  int apply_next) {
+__device__ void done_check_body(DevScalars *sc, int next_iter, double *log, int log_cap, int set_iter,
+                                int apply_next) {
   if (sc->done) return;
   // updatePenaltyFactor as the iteration's last trial staged it
   // (reduce_wl_kernel, lambda_update 3 and 4)
@@ -181,6 +185,21 @@ __global__ void done_check_kernel(DevScalars *sc, int next_iter, double *log, in
   if (stop) sc->done = 1;
 }
 
+__global__ void done_check_kernel(DevScalars *sc, int next_iter, double *log, int log_cap, int set_iter,
+                                  int apply_next) {
+  done_check_body(sc, next_iter, log, log_cap, set_iter, apply_next);
+}
+
+// The end of a backtracking iteration and the step of the next one in one
+// launch: the stop check (applying the staged penalty update) by thread 0,
+// then, after the barrier, the step from the accepted trial's Lipschitz sums.
+__global__ void __launch_bounds__(kFinish) done_step_kernel(DevScalars *sc, int next_iter, double *log, int log_cap,
+                                                            int nblocks, double *partials, int lip_slot) {
+  if (threadIdx.x == 0) done_check_body(sc, next_iter, log, log_cap, 1, 1);
+  __syncthreads();
+  step_scalar_block(nblocks, partials, sc, 0, lip_slot);
+}
+
 
 static int blocks_for(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
 
@@ -215,6 +234,11 @@ void launch_nesterov_move_ex(Ctx &c, const float2 *ref, const float2 *grad, cons
   hipLaunchKernelGGL(nesterov_move_kernel, dim3(blocks_for(c.O)), dim3(kBlock), 0, c.stream, c.O, c.in_stage,
                      ref, grad, main_in, main_out, ref_unclamped_out, save_ref, save_grad, c.pos, c.w, c.h,
                      core_of(c), c.sc);
+}
+
+void launch_done_step(Ctx &c, int next_iter, double *log, int log_cap, int lip_slot) {
+  hipLaunchKernelGGL(done_step_kernel, dim3(1), dim3(kFinish), 0, c.stream, c.sc, next_iter, log, log_cap,
+                     object_blocks(c.O), c.partials, lip_slot);
 }
 
 void launch_done_check_ex(Ctx &c, int next_iter, double *log, int log_cap, int set_iter, int apply_next) {
