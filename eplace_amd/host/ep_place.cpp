@@ -2,7 +2,7 @@
 // (main/ePlace_main.cpp:35-223) over libeplace_amd.so.
 //
 //   ep_place -aux <design.aux> [-targetDensity 1.0] [-targetOverflow 0.10]
-//            [-loadpl <file.pl>] [-noQP] [-IPiteCount <n>] [-bktrk] [-bb] [-LSE]
+//            [-loadpl <file.pl>] [-noQP] [-IPiteCount <n>] [-bktrk] [-bb] [-learned <params.npy>] [-LSE]
 //            [-bins <n>] [-nomGP] [-internalLegal] [-internalDP] [-outputPath <dir>]
 //            [-logCSV <file>]
 //            [-dumpDesign <file>]
@@ -25,12 +25,16 @@
 // backtracks so far), the fields the reference prints per iteration.
 // -dumpDesign writes the parsed arrays (ep_design field order, raw
 // little-endian) to a file and exits before any device work.
+// -learned runs every stage's Nesterov loop as backtracking led by the
+// learned step / penalty controller (the parameters eplace_amd saves, a
+// float32 .npy file; no counterpart in the reference).
 #include <sys/stat.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 
 #include "bookshelf.hpp"
@@ -41,19 +45,20 @@ using namespace eplace_amd;
 namespace {
 
 struct Args {
-  std::string aux, loadpl, out = ".", dump, log_csv;
+  std::string aux, loadpl, out = ".", dump, log_csv, learned;
   float td = 1.0f, tof = 0.10f;
   bool bktrk = false, bb = false, lse = false, no_mgp = false, legal = false, dp = false, no_qp = false, no_mlg = false, no_cgp = false;
   int bins = 0, qp_iters = 20;
-  // -bb wins over -bktrk, as in EplaceNesterovOpt::opt_step (nesterov.hpp:157-165)
-  int method() const { return bb ? STEP_BB : bktrk ? STEP_BKTRK : STEP_VANILLA; }
+  // -bb wins over -bktrk, as in EplaceNesterovOpt::opt_step (nesterov.hpp:157-165);
+  // -learned (backtracking led by the controller) over both
+  int method() const { return !learned.empty() ? STEP_LEARNED : bb ? STEP_BB : bktrk ? STEP_BKTRK : STEP_VANILLA; }
 };
 
 int usage(const char *prog) {
   fprintf(stderr,
           "usage: %s -aux <design.aux> [-targetDensity d] [-targetOverflow t] [-loadpl f.pl]\n"
           "          [-noQP] [-IPiteCount n] [-bktrk] [-bb] [-LSE] [-bins n] [-nomGP] [-nomLG] [-nocGP]\n"
-          "          [-internalLegal] [-internalDP] [-logCSV file]\n"
+          "          [-learned params.npy] [-internalLegal] [-internalDP] [-logCSV file]\n"
           "          [-outputPath dir] [-dumpDesign file]\n",
           prog);
   return 2;
@@ -99,6 +104,8 @@ bool parse(int argc, char **argv, Args &a) {
       a.dp = true;
     } else if (k == "-logCSV") {
       if (!val(a.log_csv)) return false;
+    } else if (k == "-learned") {
+      if (!val(a.learned)) return false;
     } else if (k == "-IPiteCount") {
       if (!val(v)) return false;
       a.qp_iters = atoi(v.c_str());
@@ -151,6 +158,8 @@ int main(int argc, char **argv) {
   Args a;
   if (!parse(argc, argv, a)) return usage(argv[0]);
   try {
+    // read and checked before any device work
+    const std::vector<float> ctrl_params = a.learned.empty() ? std::vector<float>() : Controller::load(a.learned);
     BookshelfDesign d = read_bookshelf(a.aux);
     if (!a.loadpl.empty()) load_pl(d, a.loadpl);
     printf("design %s: %zu nodes, %zu terminals, %zu nets, %zu pins, %zu rows\n", d.name.c_str(),
@@ -181,6 +190,9 @@ int main(int argc, char **argv) {
     o.lse = a.lse;
     EPlacer_2D placer(view, a.td, o);
     placer.initialization();
+    // -learned: the controller's saved parameters, on the placer's device
+    std::unique_ptr<Controller> ctrl;
+    if (!a.learned.empty()) ctrl.reset(new Controller(ctrl_params, o.device));
     ep_info info = placer.info();
     printf("bins %dx%d, %lld fillers, target density %.3f\n", info.bins_x, info.bins_y, (long long)info.num_fillers, a.td);
 
@@ -201,6 +213,7 @@ int main(int argc, char **argv) {
     };
     if (!a.no_mgp) {
       EplaceNesterovOpt<VECTOR_3D> opt(&placer, a.method(), a.tof);
+      opt.setController(ctrl.get());
       opt.keepLog(csv != nullptr);
       opt.opt();
       log_rows("mGP", opt);
@@ -223,12 +236,14 @@ int main(int argc, char **argv) {
       if (!a.no_cgp) {
         placer.switch2FillerOnly();
         EplaceNesterovOpt<VECTOR_3D> fo(&placer, a.method(), a.tof);
+        fo.setController(ctrl.get());
         fo.keepLog(csv != nullptr);
         fo.opt();
         log_rows("FILLERONLY", fo);
         printf("filler placement finished: %d iterations\n", fo.last().iterations);
         placer.switch2cGP();
         EplaceNesterovOpt<VECTOR_3D> cg(&placer, a.method(), a.tof);
+        cg.setController(ctrl.get());
         cg.keepLog(csv != nullptr);
         cg.opt();
         log_rows("cGP", cg);

@@ -11,6 +11,9 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -194,8 +197,69 @@ class FirstOrderOptimizer {
 
 // Step-length variants of EplaceNesterovOpt::opt_step (nesterov.hpp:152-165):
 // vanilla (default), backtracking (-bktrk), Barzilai-Borwein (-bb, which
-// the reference checks first).  The values are ep_opt_params.method.
-enum StepMethod { STEP_VANILLA = 0, STEP_BKTRK = 1, STEP_BB = 3 };
+// the reference checks first), and backtracking led by the learned
+// controller (no counterpart in the reference).  The values are
+// ep_opt_params.method.
+enum StepMethod { STEP_VANILLA = 0, STEP_BKTRK = 1, STEP_LEARNED = 2, STEP_BB = 3 };
+
+// The float32 values of a little-endian, C-order .npy file (the format the
+// controller's parameters are saved in).
+inline std::vector<float> read_npy_f32(const std::string &path) {
+  FILE *f = fopen(path.c_str(), "rb");
+  if (!f) throw std::runtime_error("cannot open " + path);
+  std::unique_ptr<FILE, int (*)(FILE *)> guard(f, fclose);
+  unsigned char pre[10];
+  if (fread(pre, 1, 8, f) != 8 || memcmp(pre, "\x93NUMPY", 6) != 0) throw std::runtime_error(path + ": not a .npy file");
+  size_t hlen = 0;
+  if (pre[6] == 1) {
+    if (fread(pre + 8, 1, 2, f) != 2) throw std::runtime_error(path + ": truncated header");
+    hlen = pre[8] | (size_t)pre[9] << 8;
+  } else {
+    unsigned char l[4];
+    if (fread(l, 1, 4, f) != 4) throw std::runtime_error(path + ": truncated header");
+    hlen = l[0] | (size_t)l[1] << 8 | (size_t)l[2] << 16 | (size_t)l[3] << 24;
+  }
+  std::string header(hlen, ' ');
+  if (fread(&header[0], 1, hlen, f) != hlen) throw std::runtime_error(path + ": truncated header");
+  if (header.find("'descr': '<f4'") == std::string::npos || header.find("'fortran_order': False") == std::string::npos)
+    throw std::runtime_error(path + ": expected little-endian float32 in C order");
+  std::vector<float> v;
+  float buf[1024];
+  size_t n;
+  while ((n = fread(buf, sizeof(float), 1024, f)) > 0) v.insert(v.end(), buf, buf + n);
+  return v;
+}
+
+// The learned step / penalty controller on one device, from saved
+// parameters (EP_CTRL_PARAMS floats).
+class Controller {
+ public:
+  // The parameters of a saved controller, checked (no device work).
+  static std::vector<float> load(const std::string &npy_path) {
+    std::vector<float> p = read_npy_f32(npy_path);
+    if (p.size() != (size_t)EP_CTRL_PARAMS)
+      throw std::runtime_error(npy_path + ": expected " + std::to_string((int)EP_CTRL_PARAMS) + " parameters, found " +
+                               std::to_string(p.size()));
+    return p;
+  }
+  explicit Controller(const std::vector<float> &p, int device = 0) {
+    if (p.size() != (size_t)EP_CTRL_PARAMS) throw std::runtime_error("controller: wrong number of parameters");
+    if (ep_ctrl_create(device, p.data(), 1, &h_) != EP_OK) {
+      std::string msg = h_ ? ep_ctrl_last_error(h_) : "ep_ctrl_create failed";
+      if (h_) ep_ctrl_destroy(h_);
+      throw std::runtime_error(msg);
+    }
+  }
+  ~Controller() {
+    if (h_) ep_ctrl_destroy(h_);
+  }
+  Controller(const Controller &) = delete;
+  Controller &operator=(const Controller &) = delete;
+  const ep_ctrl *get() const { return h_; }
+
+ private:
+  ep_ctrl *h_ = nullptr;
+};
 
 // Nesterov with the chosen step length.
 template <typename T>
@@ -213,6 +277,7 @@ class EplaceNesterovOpt : public FirstOrderOptimizer<T> {
     p.log_capacity = 0;
     p.log = nullptr;
     p.phase_seconds = nullptr;
+    p.controller = controller_;
     if (keepLog_) {
       log_.assign((size_t)(p.max_iterations + 2) * EP_LOG_COLS, 0.0);
       p.log_capacity = p.max_iterations + 2;
@@ -228,6 +293,8 @@ class EplaceNesterovOpt : public FirstOrderOptimizer<T> {
   // (nesterov.hpp:154, 333, 395) plus evaluations, backtracks and the
   // controller's inputs.
   void keepLog(bool on) { keepLog_ = on; }
+  // The controller STEP_LEARNED runs with (on the placer's device).
+  void setController(const Controller *c) { controller_ = c ? c->get() : nullptr; }
   const std::vector<double> &log() const { return log_; }
 
  private:
@@ -237,6 +304,7 @@ class EplaceNesterovOpt : public FirstOrderOptimizer<T> {
   ep_run_stats last_{};
   bool keepLog_ = false;
   std::vector<double> log_;
+  const ep_ctrl *controller_ = nullptr;
 };
 
 }  // namespace eplace_amd

@@ -132,3 +132,57 @@ def test_cli_matches_python_api(tmp_path):
     xy = gp.read("positions")[: e.num_nodes]
     np.testing.assert_allclose(out.node_x, xy[:, 0], atol=1e-3)
     np.testing.assert_allclose(out.node_y, xy[:, 1], atol=1e-3)
+
+
+def test_cli_learned_parameters_checked_before_device_work(tmp_path):
+    """-learned reads the controller's saved parameters (a float32 .npy file)
+    before any device work: the shipped file is accepted, a file that is not
+    .npy or holds the wrong number of floats is refused with a message."""
+    d = synthetic(num_cells=100, num_nets=120, seed=1, name="lrn")
+    aux = write_bookshelf(d, str(tmp_path / "in"))
+    shipped = os.path.join(ROOT, "eplace_amd", "controller_params.npy")
+    r = subprocess.run([exe(), "-aux", aux, "-learned", shipped, "-dumpDesign", str(tmp_path / "d.bin")],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    bad = tmp_path / "bad.npy"
+    bad.write_bytes(b"not numpy")
+    r = subprocess.run([exe(), "-aux", aux, "-learned", str(bad)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "not a .npy file" in r.stderr
+    np.save(tmp_path / "short.npy", np.zeros(10, np.float32))
+    r = subprocess.run([exe(), "-aux", aux, "-learned", str(tmp_path / "short.npy")], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 1 and "found 10" in r.stderr
+    np.save(tmp_path / "f64.npy", np.zeros(5150, np.float64))
+    r = subprocess.run([exe(), "-aux", aux, "-learned", str(tmp_path / "f64.npy")], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 1 and "float32" in r.stderr
+
+
+@pytest.mark.gpu
+def test_cli_learned_controller_matches_python_api(tmp_path):
+    """ep_place -learned runs the same device loop as the Python API with the
+    same controller: the same iterations and final HPWL."""
+    from eplace_amd import EPlacer2D, EplaceNesterovOpt
+    from eplace_amd.controller import Controller
+
+    d = synthetic(num_cells=20000, num_nets=21000, seed=7, name="lrn20k")
+    aux = write_bookshelf(d, str(tmp_path / "in"))
+    shipped = os.path.join(ROOT, "eplace_amd", "controller_params.npy")
+    print("ep_place -learned on 20000 cells", flush=True)
+    r = subprocess.run([exe(), "-aux", aux, "-noQP", "-learned", shipped, "-outputPath", str(tmp_path / "out")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    line = [ln for ln in r.stdout.splitlines() if ln.startswith("mGP finished")][0]
+    iters, overflow = int(line.split()[2]), float(line.split()[-1])
+    hpwl_cli = float([ln for ln in r.stdout.splitlines() if ln.startswith("Final HPWL")][0].split()[-1])
+    e = read_bookshelf(aux)
+    gp = EPlacer2D(e, 1.0)
+    gp.initialization()
+    ctrl = Controller.load()
+    res = EplaceNesterovOpt(gp, method="learned", controller=ctrl, log=False).opt()
+    gp.close()
+    print(f"  cli {iters} iterations, overflow {overflow:.4f}, HPWL {hpwl_cli:.6e}; python {res['iterations']}, "
+          f"{res['final_hpwl']:.6e}", flush=True)
+    assert overflow <= 0.10
+    assert iters == res["iterations"]
+    assert abs(hpwl_cli - res["final_hpwl"]) <= 1e-6 * res["final_hpwl"]
