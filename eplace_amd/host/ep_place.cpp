@@ -28,6 +28,8 @@
 // -learned runs every stage's Nesterov loop as backtracking led by the
 // learned step / penalty controller (the parameters eplace_amd saves, a
 // float32 .npy file; no counterpart in the reference).
+// -poissonMode selects the transforms' arithmetic (ep_params.poisson_mode:
+// 0 FP32 MFMA, the default; 2 split-FP16, the fast mode).
 #include <sys/stat.h>
 
 #include <algorithm>
@@ -48,7 +50,7 @@ struct Args {
   std::string aux, loadpl, out = ".", dump, log_csv, learned;
   float td = 1.0f, tof = 0.10f;
   bool bktrk = false, bb = false, lse = false, no_mgp = false, legal = false, dp = false, no_qp = false, no_mlg = false, no_cgp = false;
-  int bins = 0, qp_iters = 20;
+  int bins = 0, qp_iters = 20, poisson_mode = 0;
   // -bb wins over -bktrk, as in EplaceNesterovOpt::opt_step (nesterov.hpp:157-165);
   // -learned (backtracking led by the controller) over both
   int method() const { return !learned.empty() ? STEP_LEARNED : bb ? STEP_BB : bktrk ? STEP_BKTRK : STEP_VANILLA; }
@@ -58,7 +60,7 @@ int usage(const char *prog) {
   fprintf(stderr,
           "usage: %s -aux <design.aux> [-targetDensity d] [-targetOverflow t] [-loadpl f.pl]\n"
           "          [-noQP] [-IPiteCount n] [-bktrk] [-bb] [-LSE] [-bins n] [-nomGP] [-nomLG] [-nocGP]\n"
-          "          [-learned params.npy] [-internalLegal] [-internalDP] [-logCSV file]\n"
+          "          [-learned params.npy] [-poissonMode m] [-internalLegal] [-internalDP] [-logCSV file]\n"
           "          [-outputPath dir] [-dumpDesign file]\n",
           prog);
   return 2;
@@ -106,6 +108,9 @@ bool parse(int argc, char **argv, Args &a) {
       if (!val(a.log_csv)) return false;
     } else if (k == "-learned") {
       if (!val(a.learned)) return false;
+    } else if (k == "-poissonMode") {
+      if (!val(v)) return false;
+      a.poisson_mode = atoi(v.c_str());
     } else if (k == "-IPiteCount") {
       if (!val(v)) return false;
       a.qp_iters = atoi(v.c_str());
@@ -188,6 +193,7 @@ int main(int argc, char **argv) {
     PlacerOptions o;
     o.bins_x = o.bins_y = a.bins;
     o.lse = a.lse;
+    o.poisson_mode = a.poisson_mode;
     EPlacer_2D placer(view, a.td, o);
     placer.initialization();
     // -learned: the controller's saved parameters, on the placer's device

@@ -33,6 +33,7 @@ struct PlacerOptions {
   bool lse = false;            // -LSE
   bool true_exp = false;       // exp instead of (1+a/1024)^1024
   uint32_t filler_seed = 1;
+  int poisson_mode = 0;        // ep_params.poisson_mode (0 FP32 MFMA, 2 split-FP16, ...)
   int device = 0;
 };
 
@@ -48,7 +49,7 @@ class EPlacer_2D {
     p.filler_seed = o.filler_seed;
     p.filler_xy = nullptr;
     p.filler_count = -1;
-    p.poisson_mode = 0;
+    p.poisson_mode = o.poisson_mode;
     p.device = o.device;
     int rc = ep_create(&design, &p, &ctx_);
     if (rc != EP_OK) {

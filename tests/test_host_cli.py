@@ -87,6 +87,15 @@ def test_cli_usage_and_bad_input(tmp_path):
     assert r.returncode == 1 and "cannot open" in r.stderr
 
 
+def test_cli_rejects_unknown_poisson_mode(tmp_path):
+    """An unknown -poissonMode is refused by ep_create's argument checks."""
+    d = synthetic(num_cells=100, num_nets=120, seed=1, name="pm")
+    aux = write_bookshelf(d, str(tmp_path / "in"))
+    r = subprocess.run([exe(), "-aux", aux, "-noQP", "-poissonMode", "7", "-outputPath", str(tmp_path / "out")],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "poisson_mode must be" in r.stderr
+
+
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="checks the no-device error path")
 def test_cli_without_device_fails_cleanly(tmp_path):
     d = synthetic(num_cells=100, num_nets=120, seed=1, name="nodev")
@@ -160,8 +169,9 @@ def test_cli_learned_parameters_checked_before_device_work(tmp_path):
 
 @pytest.mark.gpu
 def test_cli_learned_controller_matches_python_api(tmp_path):
-    """ep_place -learned runs the same device loop as the Python API with the
-    same controller: the same iterations and final HPWL."""
+    """ep_place -learned -poissonMode 2 runs the same device loop as the
+    Python API with the same controller and split-FP16 transforms: the same
+    iterations and final HPWL."""
     from eplace_amd import EPlacer2D, EplaceNesterovOpt
     from eplace_amd.controller import Controller
 
@@ -169,14 +179,14 @@ def test_cli_learned_controller_matches_python_api(tmp_path):
     aux = write_bookshelf(d, str(tmp_path / "in"))
     shipped = os.path.join(ROOT, "eplace_amd", "controller_params.npy")
     print("ep_place -learned on 20000 cells", flush=True)
-    r = subprocess.run([exe(), "-aux", aux, "-noQP", "-learned", shipped, "-outputPath", str(tmp_path / "out")],
-                       capture_output=True, text=True, timeout=120)
+    r = subprocess.run([exe(), "-aux", aux, "-noQP", "-learned", shipped, "-poissonMode", "2", "-outputPath",
+                        str(tmp_path / "out")], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     line = [ln for ln in r.stdout.splitlines() if ln.startswith("mGP finished")][0]
     iters, overflow = int(line.split()[2]), float(line.split()[-1])
     hpwl_cli = float([ln for ln in r.stdout.splitlines() if ln.startswith("Final HPWL")][0].split()[-1])
     e = read_bookshelf(aux)
-    gp = EPlacer2D(e, 1.0)
+    gp = EPlacer2D(e, 1.0, poisson_mode=2)
     gp.initialization()
     ctrl = Controller.load()
     res = EplaceNesterovOpt(gp, method="learned", controller=ctrl, log=False).opt()
