@@ -130,6 +130,39 @@ __global__ void __launch_bounds__(kSortThreads) tile_hist_kernel(
   }
 }
 
+// Pass 1 fused with the Nesterov move that produces the positions: block b
+// moves the stage's objects of [b per, (b+1) per) (unless the run is done)
+// and counts the keys of all of them, as tile_hist_kernel does from the
+// stored positions.  The evaluation that follows starts at pass 2.
+__global__ void __launch_bounds__(kSortThreads) move_hist_kernel(
+    MoveArgs a, const DevScalars *sc, int O, int per, float2 *pos, const float *__restrict__ w,
+    const float *__restrict__ h, const uint8_t *__restrict__ kind, const uint8_t *__restrict__ is_big, Grid g,
+    int tyc, int ntiles, int nkeys, uint32_t *okey, uint32_t *bcount, uint32_t *ktotal, int64_t nb,
+    unsigned long long *node_q, unsigned long long *filler_q) {
+  extern __shared__ uint32_t hist[];
+  for (int k = threadIdx.x; k < nkeys; k += blockDim.x) hist[k] = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += (int64_t)gridDim.x * blockDim.x) {
+    node_q[i] = 0;
+    filler_q[i] = 0;
+  }
+  const bool moving = !sc->done;
+  const float step = (float)sc->step, coef = (float)sc->coef;
+  __syncthreads();
+  const int o0 = blockIdx.x * per, o1 = min(O, o0 + per);
+  for (int o = o0 + threadIdx.x; o < o1; o += blockDim.x) {
+    const float wo = w[o], ho = h[o];
+    const float2 p = moving && a.in_stage[o] ? move_object(a, o, pos, wo, ho, step, coef) : pos[o];
+    const uint32_t k = tile_key(p, wo, ho, kind[o], is_big[o], g, tyc, ntiles);
+    okey[o] = k;
+    atomicAdd(&hist[k], 1u);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < nkeys; k += blockDim.x) {
+    const uint32_t n = hist[k];
+    bcount[(size_t)blockIdx.x * nkeys + k] = n ? atomicAdd(&ktotal[k], n) : 0u;
+  }
+}
+
 // Pass 2: every block scans the key totals into the first slot of each key
 // (the same result in every block) and places its objects at that slot + its
 // offset inside the key + an LDS cursor.  Block 0 also writes the chunks of
@@ -565,15 +598,25 @@ __global__ void __launch_bounds__(kFinish) lambda_init_finish_kernel(int nblocks
 // ---------------------------------------------------------------------------
 static int blocks_for(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
 
-void launch_scatter(Ctx &c) {
+void launch_move_hist(Ctx &c, const MoveArgs &a) {
+  TileSort &t = c.tiles;
+  const int64_t nb = (int64_t)c.nx * c.ny;
+  const size_t lds = (size_t)t.nkeys * sizeof(uint32_t);
+  hipLaunchKernelGGL(move_hist_kernel, dim3(t.nblocks), dim3(kSortThreads), lds, c.stream, a, c.sc, c.O, t.per,
+                     c.pos, c.w, c.h, c.kind, c.is_big, c.grid(), t.tyc, t.ntiles, t.nkeys, t.okey, t.bcount,
+                     t.ktotal, nb, c.node_q, c.filler_q);
+}
+
+void launch_scatter(Ctx &c, bool counted) {
   Grid g = c.grid();
   double scale = ldexp(1.0, c.shift);
   TileSort &t = c.tiles;
   const int64_t nb = (int64_t)c.nx * c.ny;
   const size_t lds = (size_t)t.nkeys * sizeof(uint32_t);
-  hipLaunchKernelGGL(tile_hist_kernel, dim3(t.nblocks), dim3(kSortThreads), lds, c.stream, c.O, t.per, c.pos, c.w,
-                     c.h, c.kind, c.is_big, g, t.tyc, t.ntiles, t.nkeys, t.okey, t.bcount, t.ktotal, nb, c.node_q,
-                     c.filler_q);
+  if (!counted)
+    hipLaunchKernelGGL(tile_hist_kernel, dim3(t.nblocks), dim3(kSortThreads), lds, c.stream, c.O, t.per, c.pos,
+                       c.w, c.h, c.kind, c.is_big, g, t.tyc, t.ntiles, t.nkeys, t.okey, t.bcount, t.ktotal, nb,
+                       c.node_q, c.filler_q);
   hipLaunchKernelGGL(tile_place_kernel, dim3(t.nblocks), dim3(kSortThreads), lds, c.stream, c.O, t.per, t.nkeys,
                      t.okey, t.bcount, t.ktotal, c.pos, c.w, c.h, t.sobj, t.svals, t.chunks, t.nchunks);
   hipLaunchKernelGGL(tile_scatter_kernel, dim3(t.grid), dim3(kBlock), 0, c.stream, t.chunks, t.nchunks, t.svals, g,

@@ -530,9 +530,9 @@ void destroy(Ctx &c) {
 // to gtot_out with the positions to pos_out (see gather_combine_kernel).
 static void enqueue_evaluation(Ctx &c, bool lambda_update, bool lipschitz, const float2 *ref_now,
                                const float2 *ref_prev, const float2 *grad_prev, int next_lambda, float2 *pos_out,
-                               float2 *gtot_out, hipEvent_t *ev) {
+                               float2 *gtot_out, bool counted, hipEvent_t *ev) {
   if (ev) (void)hipEventRecord(ev[0], c.stream);
-  launch_scatter(c);
+  launch_scatter(c, counted);
   if (ev) (void)hipEventRecord(ev[1], c.stream);
   launch_finalize_density(c);
   if (ev) (void)hipEventRecord(ev[2], c.stream);
@@ -562,11 +562,11 @@ static void enqueue_evaluation(Ctx &c, bool lambda_update, bool lipschitz, const
 // the join).  nullptr when capture is not available.
 static hipGraphExec_t eval_graph(Ctx &c, bool lambda_update, bool lipschitz, const float2 *ref_now,
                                  const float2 *ref_prev, const float2 *grad_prev, int next_lambda, float2 *pos_out,
-                                 float2 *gtot_out) {
+                                 float2 *gtot_out, bool counted) {
   for (const Ctx::EvalGraph &g : c.eval_graphs)
     if (g.lambda_update == lambda_update && g.lipschitz == lipschitz && g.ref_now == ref_now &&
         g.ref_prev == ref_prev && g.grad_prev == grad_prev && g.next_lambda == next_lambda &&
-        g.pos_out == pos_out && g.gtot_out == gtot_out)
+        g.pos_out == pos_out && g.gtot_out == gtot_out && g.counted == counted)
       return g.exec;
   if (!c.graphs_ok) return nullptr;
   hipGraph_t graph = nullptr;
@@ -576,7 +576,7 @@ static hipGraphExec_t eval_graph(Ctx &c, bool lambda_update, bool lipschitz, con
     return nullptr;
   }
   enqueue_evaluation(c, lambda_update, lipschitz, ref_now, ref_prev, grad_prev, next_lambda, pos_out, gtot_out,
-                     nullptr);
+                     counted, nullptr);
   hipError_t e = hipStreamEndCapture(c.stream, &graph);
   if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   if (graph) (void)hipGraphDestroy(graph);
@@ -586,7 +586,7 @@ static hipGraphExec_t eval_graph(Ctx &c, bool lambda_update, bool lipschitz, con
     return nullptr;
   }
   c.eval_graphs.push_back({lambda_update, lipschitz, ref_now, ref_prev, grad_prev, next_lambda, pos_out, gtot_out,
-                           exec});
+                           counted, exec});
   return exec;
 }
 
@@ -595,7 +595,7 @@ static hipGraphExec_t eval_graph(Ctx &c, bool lambda_update, bool lipschitz, con
 // timed.
 void evaluate(Ctx &c, bool lambda_update, bool lipschitz, const float2 *ref_now, const float2 *ref_prev,
               const float2 *grad_prev, bool timed, int next_lambda = 0, float2 *pos_out = nullptr,
-              float2 *gtot_out = nullptr) {
+              float2 *gtot_out = nullptr, bool counted = false) {
   hipEvent_t *ev = nullptr;
   if (timed) {
     ev = c.ev;
@@ -605,14 +605,15 @@ void evaluate(Ctx &c, bool lambda_update, bool lipschitz, const float2 *ref_now,
   }
   if (!ev) {
     hipGraphExec_t g = eval_graph(c, lambda_update, lipschitz, ref_now, ref_prev, grad_prev, next_lambda, pos_out,
-                                  gtot_out);
+                                  gtot_out, counted);
     if (g && hipGraphLaunch(g, c.stream) == hipSuccess) return;
   }
   // profiled: the device waits until the whole evaluation is enqueued, so the
   // phases run back to back as in a graph replay and the events time kernels,
   // not the host's submissions
   if (ev && !timed) launch_hold(c, c.prof_hold_ticks);
-  enqueue_evaluation(c, lambda_update, lipschitz, ref_now, ref_prev, grad_prev, next_lambda, pos_out, gtot_out, ev);
+  enqueue_evaluation(c, lambda_update, lipschitz, ref_now, ref_prev, grad_prev, next_lambda, pos_out, gtot_out,
+                     counted, ev);
 }
 
 // Phase times of one evaluation from its events: scatter, finalise, wiring
@@ -815,6 +816,8 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
         if (k == 0 && c.eval_valid) launch_gather_combine(c, false, true, nullptr, c.last_ref, c.last_grad);
         else evaluate(c, k > 0, true, nullptr, c.last_ref, c.last_grad, false);
         launch_step_scalar(c, bb ? 1 : 0);
+        // (the last move of a run has no evaluation after it, so the keys are
+        // counted by the evaluation itself)
         launch_nesterov_move_ex(c, nullptr, c.gtot, c.cur_main, c.cur_main, nullptr, c.last_ref, c.last_grad);
         launch_done_check_ex(c, k + 1, dlog, cap, 1);
       }
@@ -840,10 +843,14 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
       int tries = 0;
       bool stopped = false;
       while (true) {
-        launch_nesterov_move_ex(c, c.cur_ref, c.cur_grad, c.cur_main, c.new_main, c.new_ref, nullptr, nullptr);
+        // every trial's move is followed by its evaluation, so the move also
+        // counts the keys of the scatter (the sort's totals are zero again
+        // only after that evaluation's scatter)
+        launch_nesterov_move_hist(c, c.cur_ref, c.cur_grad, c.cur_main, c.new_main, c.new_ref, nullptr, nullptr);
         // y_k, g_k of the previous iteration (last_ref, last_grad) are free:
         // the trial's positions and its gradient at the staged lambda go there
-        evaluate(c, false, true, c.new_ref, c.cur_ref, c.cur_grad, false, learned ? 2 : 1, c.last_ref, c.last_grad);
+        evaluate(c, false, true, c.new_ref, c.cur_ref, c.cur_grad, false, learned ? 2 : 1, c.last_ref, c.last_grad,
+                 true);
         launch_accept(c);
         DevScalars h;
         if ((rc = read_scalars(c, h))) return rc;

@@ -94,6 +94,49 @@ __device__ __forceinline__ int xcd_block() {
   return (b % kXcds) * (n / kXcds) + b / kXcds;
 }
 
+// The core region, for the position clamp.
+struct Core {
+  float llx, lly, urx, ury;
+};
+
+// PlaceDB::setModuleCenter_2D clamp (placedb.cpp:253-288): compares in double,
+// +/- EPS, result stored as float.
+__device__ __forceinline__ float2 clamp_center(float x, float y, float w, float h, const Core &k) {
+  double hw = 0.5 * (double)w, hh = 0.5 * (double)h;
+  if ((double)x - hw < (double)k.llx) x = (float)((double)k.llx + hw + kEps);
+  if ((double)x + hw > (double)k.urx) x = (float)((double)k.urx - hw - kEps);
+  if ((double)y - hh < (double)k.lly) y = (float)((double)k.lly + hh + kEps);
+  if ((double)y + hh > (double)k.ury) y = (float)((double)k.ury - hh - kEps);
+  return make_float2(x, y);
+}
+
+// One Nesterov move (nesterov.hpp:400-411 + setPosition):
+// x_{k+1} = y_k + step g_k;  y_{k+1} = x_{k+1} + coef (x_{k+1} - x_k);
+// positions := clamp(y_{k+1}).  ref == nullptr reads y_k from the
+// positions; save_ref/save_grad receive y_k and g_k for the next Lipschitz
+// estimate.
+struct MoveArgs {
+  const uint8_t *in_stage;
+  const float2 *ref, *grad, *main_in;
+  float2 *main_out, *ref_unclamped_out, *save_ref, *save_grad;
+  Core k;
+};
+__device__ __forceinline__ float2 move_object(const MoveArgs &a, int o, float2 *pos, float w, float h, float step,
+                                              float coef) {
+  const float2 r = a.ref ? a.ref[o] : pos[o];
+  const float2 g = a.grad[o];
+  const float2 m = a.main_in[o];
+  if (a.save_ref) a.save_ref[o] = r;
+  if (a.save_grad) a.save_grad[o] = g;
+  const float npx = r.x + g.x * step, npy = r.y + g.y * step;
+  const float nrx = npx + (npx - m.x) * coef, nry = npy + (npy - m.y) * coef;
+  a.main_out[o] = make_float2(npx, npy);
+  if (a.ref_unclamped_out) a.ref_unclamped_out[o] = make_float2(nrx, nry);
+  const float2 p = clamp_center(nrx, nry, w, h, a.k);
+  pos[o] = p;
+  return p;
+}
+
 // Blocks of kBlock threads covering n items, rounded up to a multiple of
 // kXcds so that xcd_block() applies (the extra blocks find no work).
 inline int object_blocks(int64_t n) {
@@ -212,6 +255,7 @@ struct Ctx {
     const float2 *ref_now, *ref_prev, *grad_prev;
     int next_lambda;
     float2 *pos_out, *gtot_out;
+    bool counted;  // the keys were counted by the move before it
     hipGraphExec_t exec;
   };
   std::vector<EvalGraph> eval_graphs;
@@ -305,7 +349,11 @@ struct Ctx {
 };
 
 // ---- launchers (density.hip) ----
-void launch_scatter(Ctx &c);
+// counted: the keys were counted and the maps cleared by launch_move_hist
+void launch_scatter(Ctx &c, bool counted = false);
+// The Nesterov move of the stage's objects fused with the scatter's key count
+// (the evaluation that follows calls launch_scatter(c, true)).
+void launch_move_hist(Ctx &c, const MoveArgs &a);
 void launch_finalize_density(Ctx &c);
 // pos_out / gtot_out (optional): copies of the positions and of the total
 // gradient written by the same pass
@@ -349,6 +397,9 @@ void launch_done_check_ex(Ctx &c, int next_iter, double *log, int log_cap, int s
 // in one launch
 void launch_done_step(Ctx &c, int next_iter, double *log, int log_cap, int lip_slot);
 void launch_nesterov_move_ex(Ctx &c, const float2 *ref, const float2 *grad, const float2 *main_in,
+                             float2 *main_out, float2 *ref_unclamped_out, float2 *save_ref, float2 *save_grad);
+// the same move with the next evaluation's key count (launch_move_hist)
+void launch_nesterov_move_hist(Ctx &c, const float2 *ref, const float2 *grad, const float2 *main_in,
                              float2 *main_out, float2 *ref_unclamped_out, float2 *save_ref, float2 *save_grad);
 int poisson_ld(const Ctx &c);
 // ---- controller.hip ----

@@ -7,21 +7,6 @@
 
 namespace ep {
 
-struct Core {
-  float llx, lly, urx, ury;
-};
-
-// PlaceDB::setModuleCenter_2D clamp (placedb.cpp:253-288): compares in double,
-// +/- EPS, result stored as float.
-__device__ __forceinline__ float2 clamp_center(float x, float y, float w, float h, const Core &k) {
-  double hw = 0.5 * (double)w, hh = 0.5 * (double)h;
-  if ((double)x - hw < (double)k.llx) x = (float)((double)k.llx + hw + kEps);
-  if ((double)x + hw > (double)k.urx) x = (float)((double)k.urx - hw - kEps);
-  if ((double)y - hh < (double)k.lly) y = (float)((double)k.lly + hh + kEps);
-  if ((double)y + hh > (double)k.ury) y = (float)((double)k.ury - hh - kEps);
-  return make_float2(x, y);
-}
-
 __global__ void set_positions_kernel(int64_t n, const float2 *__restrict__ xy, const int32_t *__restrict__ idx,
                                      float2 *pos, const float *__restrict__ w, const float *__restrict__ h,
                                      Core k) {
@@ -115,28 +100,14 @@ __global__ void __launch_bounds__(kFinish) accept_kernel(int nblocks, double *pa
   }
 }
 
-// x_{k+1} = y_k + step g_k;  y_{k+1} = x_{k+1} + coef (x_{k+1} - x_k);
-// positions := clamp(y_{k+1}) (nesterov.hpp:400-411 + setPosition).
-// ref == nullptr reads y_k from the positions; save_ref/save_grad receive
-// y_k and g_k for the next Lipschitz estimate.
-__global__ void __launch_bounds__(kBlock) nesterov_move_kernel(
-    int O, const uint8_t *__restrict__ in_stage, const float2 *ref, const float2 *__restrict__ grad,
-    const float2 *main_in, float2 *main_out, float2 *ref_unclamped_out, float2 *save_ref, float2 *save_grad,
-    float2 *pos, const float *__restrict__ w, const float *__restrict__ h, Core k, const DevScalars *sc) {
+// The move of every object of the stage (move_object).
+__global__ void __launch_bounds__(kBlock) nesterov_move_kernel(MoveArgs a, int O, float2 *pos,
+                                                               const float *__restrict__ w,
+                                                               const float *__restrict__ h, const DevScalars *sc) {
   if (sc->done) return;
   int o = blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= O || !in_stage[o]) return;
-  float step = (float)sc->step, coef = (float)sc->coef;
-  float2 r = ref ? ref[o] : pos[o];
-  float2 g = grad[o];
-  float2 m = main_in[o];
-  if (save_ref) save_ref[o] = r;
-  if (save_grad) save_grad[o] = g;
-  float npx = r.x + g.x * step, npy = r.y + g.y * step;
-  float nrx = npx + (npx - m.x) * coef, nry = npy + (npy - m.y) * coef;
-  main_out[o] = make_float2(npx, npy);
-  if (ref_unclamped_out) ref_unclamped_out[o] = make_float2(nrx, nry);
-  pos[o] = clamp_center(nrx, nry, w[o], h[o], k);
+  if (o >= O || !a.in_stage[o]) return;
+  move_object(a, o, pos, w[o], h[o], (float)sc->step, (float)sc->coef);
 }
 
 // End of an iteration: commit a_{k+1}, advance the counter, evaluate the
@@ -231,14 +202,20 @@ void launch_accept(Ctx &c) {
 
 void launch_nesterov_move_ex(Ctx &c, const float2 *ref, const float2 *grad, const float2 *main_in,
                              float2 *main_out, float2 *ref_unclamped_out, float2 *save_ref, float2 *save_grad) {
-  hipLaunchKernelGGL(nesterov_move_kernel, dim3(blocks_for(c.O)), dim3(kBlock), 0, c.stream, c.O, c.in_stage,
-                     ref, grad, main_in, main_out, ref_unclamped_out, save_ref, save_grad, c.pos, c.w, c.h,
-                     core_of(c), c.sc);
+  const MoveArgs a{c.in_stage, ref, grad, main_in, main_out, ref_unclamped_out, save_ref, save_grad, core_of(c)};
+  hipLaunchKernelGGL(nesterov_move_kernel, dim3(blocks_for(c.O)), dim3(kBlock), 0, c.stream, a, c.O, c.pos, c.w,
+                     c.h, c.sc);
 }
 
 void launch_done_step(Ctx &c, int next_iter, double *log, int log_cap, int lip_slot) {
   hipLaunchKernelGGL(done_step_kernel, dim3(1), dim3(kFinish), 0, c.stream, c.sc, next_iter, log, log_cap,
                      object_blocks(c.O), c.partials, lip_slot);
+}
+
+void launch_nesterov_move_hist(Ctx &c, const float2 *ref, const float2 *grad, const float2 *main_in,
+                               float2 *main_out, float2 *ref_unclamped_out, float2 *save_ref, float2 *save_grad) {
+  launch_move_hist(c, MoveArgs{c.in_stage, ref, grad, main_in, main_out, ref_unclamped_out, save_ref, save_grad,
+                               core_of(c)});
 }
 
 void launch_done_check_ex(Ctx &c, int next_iter, double *log, int log_cap, int set_iter, int apply_next) {
