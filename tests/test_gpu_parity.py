@@ -96,6 +96,24 @@ def test_scatter_and_gather_after_large_moves(case):
     assert e < 1e-5
 
 
+@pytest.mark.parametrize("method", ["bktrk", "vanilla"])
+def test_density_exact_after_a_run(method):
+    """After a Nesterov run (whose backtracking trials count the scatter's
+    keys inside their moves) a plain evaluation at the final positions still
+    gives the integer oracle's maps bit for bit: the sort's key totals were
+    left at zero."""
+    say(f"[density] after a {method} run")
+    d, gp, orc = make("c2_small_64")
+    gp.initialization()
+    EplaceNesterovOpt(gp, method=method, max_iterations=6, log=False).opt()
+    gp.totalGradientUpdate()
+    orc.set("positions", gp.read("positions").reshape(-1).astype(np.float64))
+    orc.run("scatter_fixed", gp.info()["fixed_shift"])
+    np.testing.assert_array_equal(gp.read("node_q").reshape(-1), orc.get("node_q"))
+    np.testing.assert_array_equal(gp.read("filler_q").reshape(-1), orc.get("filler_q"))
+    gp.close()
+
+
 POISSON_GRIDS = [(16, 16), (10, 6), (64, 64), (128, 64), (96, 160), (256, 256), (512, 512), (1024, 1024),
                  (2048, 2048)]
 # poisson_mode: 0 FP32 MFMA, 1 split-BF16 (2 planes, 3 products), 2 split-FP16
