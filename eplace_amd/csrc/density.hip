@@ -598,6 +598,17 @@ __global__ void __launch_bounds__(kFinish) lambda_init_finish_kernel(int nblocks
 // ---------------------------------------------------------------------------
 static int blocks_for(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
 
+__global__ void clear_u32_kernel(uint32_t *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+void launch_clear_key_totals(Ctx &c) {
+  TileSort &t = c.tiles;
+  hipLaunchKernelGGL(clear_u32_kernel, dim3((t.nkeys + kBlock - 1) / kBlock), dim3(kBlock), 0, c.stream, t.ktotal,
+                     t.nkeys);
+}
+
 void launch_move_hist(Ctx &c, const MoveArgs &a) {
   TileSort &t = c.tiles;
   const int64_t nb = (int64_t)c.nx * c.ny;

@@ -813,12 +813,11 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
       for (int j = 0; j < chunk; ++j, ++k) {
         // evaluation at y_k: iteration 0 reuses the caller's evaluation at
         // these positions (only lambda changed since), later ones recompute
+        // (from the second iteration on, the previous move counted the keys)
         if (k == 0 && c.eval_valid) launch_gather_combine(c, false, true, nullptr, c.last_ref, c.last_grad);
-        else evaluate(c, k > 0, true, nullptr, c.last_ref, c.last_grad, false);
+        else evaluate(c, k > 0, true, nullptr, c.last_ref, c.last_grad, false, 0, nullptr, nullptr, k > 0);
         launch_step_scalar(c, bb ? 1 : 0);
-        // (the last move of a run has no evaluation after it, so the keys are
-        // counted by the evaluation itself)
-        launch_nesterov_move_ex(c, nullptr, c.gtot, c.cur_main, c.cur_main, nullptr, c.last_ref, c.last_grad);
+        launch_nesterov_move_hist(c, nullptr, c.gtot, c.cur_main, c.cur_main, nullptr, c.last_ref, c.last_grad);
         launch_done_check_ex(c, k + 1, dlog, cap, 1);
       }
     } else {
@@ -881,9 +880,15 @@ int nesterov_run(Ctx &c, const ep_opt_params *op, ep_run_stats *out) {
       continue;
     }
     DevScalars h;
-    if ((rc = read_scalars(c, h))) return rc;
+    if ((rc = read_scalars(c, h))) {
+      if (!bk) launch_clear_key_totals(c);
+      return rc;
+    }
     if (h.done) break;
   }
+  // the vanilla run's last move counted keys for an evaluation that does not
+  // follow: the sort's totals go back to zero
+  if (!bk && k > 0) launch_clear_key_totals(c);
   // lambda update pending after the last move (vanilla evaluates it lazily)
   DevScalars h;
   if ((rc = read_scalars(c, h))) return rc;

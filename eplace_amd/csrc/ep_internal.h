@@ -354,6 +354,8 @@ void launch_scatter(Ctx &c, bool counted = false);
 // The Nesterov move of the stage's objects fused with the scatter's key count
 // (the evaluation that follows calls launch_scatter(c, true)).
 void launch_move_hist(Ctx &c, const MoveArgs &a);
+// Zeroes the sort's key totals that a move counted for no evaluation.
+void launch_clear_key_totals(Ctx &c);
 void launch_finalize_density(Ctx &c);
 // pos_out / gtot_out (optional): copies of the positions and of the total
 // gradient written by the same pass
